@@ -1,0 +1,298 @@
+/*
+ * Oracle (test infrastructure only): random streams, channel, LLRs and statistics,
+ * restated from the reference in f64 with the platform libm (the libm Rust calls on Linux).
+ */
+#include <math.h>
+#include <string.h>
+#include <stdlib.h>
+#include "oracle.h"
+
+/* crates/com-core/src/rng.rs:49-55 */
+uint64_t or_splitmix64(uint64_t* state) {
+    *state += 0x9E3779B97F4A7C15ULL;
+    uint64_t z = *state;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+/* rng.rs:58-65 */
+uint64_t or_fnv1a64(const char* bytes, size_t n) {
+    uint64_t h = 0xcbf29ce484222325ULL;
+    for (size_t i = 0; i < n; ++i) {
+        h ^= (uint8_t)bytes[i];
+        h *= 0x00000100000001B3ULL;
+    }
+    return h;
+}
+
+/* rng.rs:75-79: four SplitMix64 outputs */
+void or_rng_new(or_rng* r, uint64_t seed) {
+    uint64_t st = seed;
+    for (int i = 0; i < 4; ++i) r->s[i] = or_splitmix64(&st);
+}
+
+static inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+/* rng.rs:98-111 (xoshiro256**) */
+uint64_t or_rng_next(or_rng* r) {
+    uint64_t* s = r->s;
+    uint64_t result = rotl(s[1] * 5, 7) * 9;
+    uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return result;
+}
+
+/* rng.rs:115-117 */
+double or_rng_uniform(or_rng* r) {
+    return (double)(or_rng_next(r) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+/* rng.rs:146-148 */
+uint8_t or_rng_bit(or_rng* r) { return (uint8_t)(or_rng_next(r) >> 63); }
+
+static const double OR_PI = 3.14159265358979323846;
+
+/* rng.rs:152-157: Box-Muller, cosine branch */
+double or_rng_gaussian(or_rng* r) {
+    double u1 = or_rng_uniform(r);
+    double u2 = or_rng_uniform(r);
+    double rad = sqrt(-2.0 * log(1.0 - u1));
+    return rad * cos(2.0 * OR_PI * u2);
+}
+
+/* rng.rs:161-167 */
+void or_rng_complex_gaussian(or_rng* r, double* re, double* im) {
+    double u1 = or_rng_uniform(r);
+    double u2 = or_rng_uniform(r);
+    double rad = sqrt(-2.0 * log(1.0 - u1)) * 0.70710678118654752440;
+    double a = 2.0 * OR_PI * u2;
+    double s = sin(a), c = cos(a);
+    *re = rad * c;
+    *im = rad * s;
+}
+
+/* rng.rs:270-278 */
+void or_stream(or_rng* r, uint64_t base, const char* name, uint64_t cond, uint64_t trial) {
+    uint64_t st = base ^ or_fnv1a64(name, strlen(name));
+    uint64_t a = or_splitmix64(&st);
+    st = a ^ cond;
+    uint64_t b = or_splitmix64(&st);
+    st = b ^ trial;
+    uint64_t c = or_splitmix64(&st);
+    or_rng_new(r, c);
+}
+
+/* crates/com-core/src/units.rs:38-40 */
+double or_db2lin(double x_db) { return pow(10.0, x_db / 10.0); }
+
+/* crates/com-fec/src/bounds.rs:58-60 */
+double or_bpsk_noise_variance(double ebn0_db, double rate) {
+    return 1.0 / (2.0 * rate * or_db2lin(ebn0_db));
+}
+
+/* crates/com-channel/src/awgn.rs:139-141 */
+double or_ebn0_noise_variance(double p, double ebn0_db, double bits, double sps) {
+    return p * sps / (bits * or_db2lin(ebn0_db));
+}
+
+/* validation/src/lib.rs:67-71 with bpsk (constellation.rs:149-151), bpsk_llr (demapper.rs:309-311) */
+void or_bpsk_awgn_llr(const uint8_t* c, int64_t n, double noise_var, or_rng* r, double* llr) {
+    double s = sqrt(noise_var);
+    for (int64_t i = 0; i < n; ++i) {
+        double x = 1.0 - 2.0 * (double)c[i];
+        double y = x + s * or_rng_gaussian(r);
+        llr[i] = 2.0 * y / noise_var;
+    }
+}
+
+/* constellation.rs:168-205 (QAM natural order, serpentine rows, Gray labels) and
+   normalized() (:396-398) with power = energy / M (com-math lib.rs:138-148) */
+void or_qam_points(int m, double* re, double* im) {
+    int l = (int)lround(sqrt((double)m));
+    double* nat_re = (double*)malloc(sizeof(double) * m);
+    double* nat_im = (double*)malloc(sizeof(double) * m);
+    int p = 0;
+    for (int row = 0; row < l; ++row) {
+        double yv = 2.0 * (double)(l - 1 - row) - ((double)l - 1.0);
+        for (int c = 0; c < l; ++c) {
+            int cc = (row % 2 == 1) ? (l - 1 - c) : c;
+            nat_re[p] = 2.0 * (double)cc - ((double)l - 1.0);
+            nat_im[p] = yv;
+            ++p;
+        }
+    }
+    for (int i = 0; i < m; ++i) {
+        int g = i ^ (i >> 1);
+        re[g] = nat_re[i];
+        im[g] = nat_im[i];
+    }
+    double e = 0.0;
+    for (int i = 0; i < m; ++i) e += re[i] * re[i] + im[i] * im[i];
+    double pw = e / (double)m;
+    double s = 1.0 / sqrt(pw);
+    for (int i = 0; i < m; ++i) {
+        re[i] = re[i] * s;
+        im[i] = im[i] * s;
+    }
+    free(nat_re);
+    free(nat_im);
+}
+
+/* demapper.rs:270-297 with LabelHalves::new/llr in max-log (demapper.rs:94-146) */
+void or_qam_llr(int m, const double* yre, const double* yim, int64_t nsym, double noise_var,
+                double* llr) {
+    double pre[256], pim[256], metric[256];
+    int b = 0;
+    while ((1 << b) < m) ++b;
+    or_qam_points(m, pre, pim);
+    double uniform = -log((double)m);
+    for (int64_t n = 0; n < nsym; ++n) {
+        for (int i = 0; i < m; ++i) {
+            double dr = yre[n] - pre[i], di = yim[n] - pim[i];
+            metric[i] = -(dr * dr + di * di) / noise_var + uniform;
+        }
+        for (int o = 0; o < b; ++o) {
+            int shift = b - 1 - o;
+            double m0 = -INFINITY, m1 = -INFINITY;
+            for (int i = 0; i < m; ++i) {
+                if (((i >> shift) & 1) == 0) m0 = fmax(m0, metric[i]);
+                else m1 = fmax(m1, metric[i]);
+            }
+            llr[n * b + o] = m0 - m1;
+        }
+    }
+}
+
+/* map_bits (constellation.rs:417-419, labels MSB first: bits.rs:68-70), add_noise (awgn.rs:94-99),
+   then the max-log LLRs */
+void or_qam_awgn_llr(int m, const uint8_t* bits, int64_t nbits, double noise_var, or_rng* r,
+                     double* llr) {
+    double pre[256], pim[256];
+    int b = 0;
+    while ((1 << b) < m) ++b;
+    or_qam_points(m, pre, pim);
+    int64_t nsym = nbits / b;
+    double* yre = (double*)malloc(sizeof(double) * (size_t)nsym);
+    double* yim = (double*)malloc(sizeof(double) * (size_t)nsym);
+    double sd = sqrt(noise_var);
+    for (int64_t s = 0; s < nsym; ++s) {
+        int idx = 0;
+        for (int k = 0; k < b; ++k) idx = (idx << 1) | (bits[s * b + k] & 1);
+        double gr, gi;
+        or_rng_complex_gaussian(r, &gr, &gi);
+        yre[s] = pre[idx] + gr * sd;
+        yim[s] = pim[idx] + gi * sd;
+    }
+    or_qam_llr(m, yre, yim, nsym, noise_var, llr);
+    free(yre);
+    free(yim);
+}
+
+/* crates/com-math/src/special.rs:180-255: Cody's rational Chebyshev erfc (CALERF) */
+double or_erfc(double x) {
+    static const double A[5] = {3.16112374387056560e00, 1.13864154151050156e02,
+                                3.77485237685302021e02, 3.20937758913846947e03,
+                                1.85777706184603153e-1};
+    static const double B[4] = {2.36012909523441209e01, 2.44024637934444173e02,
+                                1.28261652607737228e03, 2.84423683343917062e03};
+    static const double C[9] = {5.64188496988670089e-1, 8.88314979438837594e00,
+                                6.61191906371416295e01, 2.98635138197400131e02,
+                                8.81952221241769090e02, 1.71204761263407058e03,
+                                2.05107837782607147e03, 1.23033935479799725e03,
+                                2.15311535474403846e-8};
+    static const double D[8] = {1.57449261107098347e01, 1.17693950891312499e02,
+                                5.37181101862009858e02, 1.62138957456669019e03,
+                                3.29079923573345963e03, 4.36261909014324716e03,
+                                3.43936767414372164e03, 1.23033935480374942e03};
+    static const double P[6] = {3.05326634961232344e-1, 3.60344899949804439e-1,
+                                1.25781726111229246e-1, 1.60837851487422766e-2,
+                                6.58749161529837803e-4, 1.63153871373020978e-2};
+    static const double Q[5] = {2.56852019228982242e00, 1.87295284992346725e00,
+                                5.27905102951428412e-1, 6.05183413124413191e-2,
+                                2.33520497626869185e-3};
+    const double FRAC_1_SQRT_PI = 5.6418958354775628695e-1;
+    double y = fabs(x);
+    if (y <= 0.46875) {
+        double ysq = y > 1.11e-16 ? y * y : 0.0;
+        double num = A[4] * ysq, den = ysq;
+        for (int i = 0; i < 3; ++i) {
+            num = (num + A[i]) * ysq;
+            den = (den + B[i]) * ysq;
+        }
+        return 1.0 - x * (num + A[3]) / (den + B[3]);
+    }
+    double r;
+    if (y > 27.3) {
+        r = 0.0;
+    } else {
+        double rr;
+        if (y <= 4.0) {
+            double num = C[8] * y, den = y;
+            for (int i = 0; i < 7; ++i) {
+                num = (num + C[i]) * y;
+                den = (den + D[i]) * y;
+            }
+            rr = (num + C[7]) / (den + D[7]);
+        } else {
+            double ysq = 1.0 / (y * y);
+            double num = P[5] * ysq, den = ysq;
+            for (int i = 0; i < 4; ++i) {
+                num = (num + P[i]) * ysq;
+                den = (den + Q[i]) * ysq;
+            }
+            rr = (FRAC_1_SQRT_PI - ysq * (num + P[4]) / (den + Q[4])) / y;
+        }
+        double u = trunc(y * 16.0) / 16.0;
+        r = exp(-u * u) * exp(-(y - u) * (y + u)) * rr;
+    }
+    return x < 0.0 ? 2.0 - r : r;
+}
+
+/* special.rs:265-273 with search.rs:69-80 (bisection, 200 steps on [-27, 27]) */
+double or_erfcinv(double y) {
+    if (y <= 0.0) return INFINITY;
+    if (y >= 2.0) return -INFINITY;
+    double lo = -27.0, hi = 27.0;
+    for (int i = 0; i < 200; ++i) {
+        double mid = 0.5 * (lo + hi);
+        if (or_erfc(mid) > y) lo = mid;
+        else hi = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+/* special.rs:291-293 */
+double or_qfuncinv(double p) { return 1.41421356237309504880 * or_erfcinv(2.0 * p); }
+
+/* crates/com-sim/src/stats.rs:43-45 */
+double or_z_score(double confidence) {
+    double c = confidence;
+    double top = 1.0 - 1e-16;
+    if (c < 0.0) c = 0.0;
+    if (c > top) c = top;
+    return or_qfuncinv((1.0 - c) / 2.0);
+}
+
+/* stats.rs:74-87 */
+void or_wilson(uint64_t errors, uint64_t total, double confidence, double* lo, double* hi) {
+    if (total == 0) {
+        *lo = 0.0;
+        *hi = 1.0;
+        return;
+    }
+    double n = (double)total;
+    double p = (double)errors / (double)total;
+    double z = or_z_score(confidence);
+    double z2 = z * z;
+    double centre = (p + z2 / (2.0 * n)) / (1.0 + z2 / n);
+    double half = z * sqrt(p * (1.0 - p) / n + z2 / (4.0 * n * n)) / (1.0 + z2 / n);
+    double lower = errors == 0 ? 0.0 : fmax(centre - half, 0.0);
+    *lo = lower;
+    *hi = fmin(centre + half, 1.0);
+}
