@@ -1,0 +1,270 @@
+// The C ABI of include/comfec_gpu.h: handles, error reporting, and the decode entry points.
+#include "../../include/comfec_gpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "kernels.hpp"
+#include "nr_code.hpp"
+
+using namespace comfec;
+
+struct cf_code {
+    NrCode code;
+    mutable std::mutex mu;  // guards the decoder settings
+    explicit cf_code(NrCode&& c) : code(std::move(c)) {}
+    DecoderSettings settings() const {
+        std::lock_guard<std::mutex> l(mu);
+        return code.dec;
+    }
+};
+
+namespace {
+
+thread_local std::string g_error;
+
+cf_status status_of(Kind k) { return static_cast<cf_status>(static_cast<int>(k)); }
+
+template <typename F>
+cf_status guarded(F&& f) {
+    try {
+        f();
+        g_error.clear();
+        return CF_OK;
+    } catch (const Error& e) {
+        g_error = e.what();
+        return status_of(e.kind);
+    } catch (const std::bad_alloc&) {
+        g_error = "device: out of host memory";
+        return CF_DEVICE;
+    } catch (const std::exception& e) {
+        g_error = std::string("device: ") + e.what();
+        return CF_DEVICE;
+    }
+}
+
+void check_hip(hipError_t err, const char* what) {
+    if (err != hipSuccess) fail(Kind::Device, std::string(what) + ": " + hipGetErrorString(err));
+}
+
+struct WorkspaceSlot {
+    std::mutex mu;
+    Workspace ws;
+};
+
+std::mutex g_ws_mu;
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<WorkspaceSlot>> g_ws;
+
+WorkspaceSlot& workspace(int device, hipStream_t s) {
+    std::lock_guard<std::mutex> l(g_ws_mu);
+    auto& slot = g_ws[{device, s}];
+    if (!slot) slot = std::make_unique<WorkspaceSlot>();
+    return *slot;
+}
+
+int current_device() {
+    int d = 0;
+    check_hip(hipGetDevice(&d), "hipGetDevice");
+    return d;
+}
+
+DevCode dev_code(const NrCode& c, int device) {
+    const NrCode::DeviceTables& t = c.device_tables(device);
+    DevCode d;
+    d.z = int32_t(c.z);
+    d.rows_b = int32_t(c.rows_b);
+    d.cols_b = int32_t(c.cols_b);
+    d.edges_b = int32_t(c.edges_b);
+    d.n = int32_t(c.n);
+    d.k = int32_t(c.k);
+    d.k_info = int32_t(c.k_info);
+    d.e = int32_t(c.e);
+    d.row_start = t.row_start;
+    d.col = t.col;
+    d.shift = t.shift;
+    d.col_start = t.col_start;
+    d.col_edge = t.col_edge;
+    d.src_start = t.src_start;
+    d.src = t.src;
+    return d;
+}
+
+template <typename T>
+GenericArgs<T> generic_args(const cf_code* h, int device) {
+    DecoderSettings s = h->settings();
+    GenericArgs<T> a{};
+    a.code = dev_code(h->code, device);
+    a.rule = s.rule;
+    a.alpha = s.alpha;
+    a.layered = s.layered ? 1 : 0;
+    a.max_iterations = int(s.max_iterations);
+    return a;
+}
+
+template <typename T>
+void run_generic(const GenericArgs<T>& a, int device, hipStream_t stream) {
+    WorkspaceSlot& slot = workspace(device, stream);
+    std::lock_guard<std::mutex> l(slot.mu);
+    check_hip(launch_generic(a, device, stream, slot.ws), "decode launch");
+}
+
+// One codeword from host f64 buffers, decoded on the GPU in f64.
+void decode_host_f64(const cf_code* h, const double* llr, bool full, const double* apriori, double* post_host,
+                     uint8_t* bits_host, uint32_t* iterations) {
+    const NrCode& c = h->code;
+    int device = current_device();
+    hipStream_t stream = nullptr;
+    size_t in_len = full ? c.n : c.e;
+    size_t bytes = (in_len + c.k_info + c.n) * sizeof(double) + c.k_info + sizeof(uint32_t) + 64;
+    void* buf = nullptr;
+    check_hip(hipMalloc(&buf, bytes), "hipMalloc");
+    std::unique_ptr<void, void (*)(void*)> guard(buf, [](void* p) { (void)hipFree(p); });
+    double* d_llr = static_cast<double*>(buf);
+    double* d_apriori = d_llr + in_len;
+    double* d_post = d_apriori + c.k_info;
+    uint32_t* d_it = reinterpret_cast<uint32_t*>(d_post + c.n);
+    uint8_t* d_bits = reinterpret_cast<uint8_t*>(d_it + 1);
+    check_hip(hipMemcpy(d_llr, llr, in_len * sizeof(double), hipMemcpyHostToDevice), "copy in");
+    if (apriori) check_hip(hipMemcpy(d_apriori, apriori, c.k_info * sizeof(double), hipMemcpyHostToDevice), "copy in");
+    GenericArgs<double> a = generic_args<double>(h, device);
+    a.input_full = full ? 1 : 0;
+    a.llr = d_llr;
+    a.apriori = apriori ? d_apriori : nullptr;
+    a.count = 1;
+    a.bits = d_bits;
+    a.post_out = d_post;
+    a.iterations = d_it;
+    run_generic(a, device, stream);
+    check_hip(hipStreamSynchronize(stream), "decode");
+    uint32_t it = 0;
+    check_hip(hipMemcpy(&it, d_it, sizeof it, hipMemcpyDeviceToHost), "copy out");
+    if (iterations) *iterations = it;
+    if (post_host) check_hip(hipMemcpy(post_host, d_post, c.n * sizeof(double), hipMemcpyDeviceToHost), "copy out");
+    if (bits_host) check_hip(hipMemcpy(bits_host, d_bits, c.k_info, hipMemcpyDeviceToHost), "copy out");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cf_last_error(void) { return g_error.c_str(); }
+
+cf_status cf_nr_create(uint32_t k_info, uint32_t e, int base_graph, int rv, uint32_t qm, cf_code** out) {
+    return guarded([&] {
+        if (!out) fail(Kind::Config, "null output handle");
+        *out = new cf_code(NrCode::with_options(k_info, e, base_graph, rv, qm));
+    });
+}
+
+cf_status cf_lifted_create(int base_graph, uint32_t z, cf_code** out) {
+    return guarded([&] {
+        if (!out) fail(Kind::Config, "null output handle");
+        *out = new cf_code(NrCode::lifted(base_graph, z));
+    });
+}
+
+void cf_code_destroy(cf_code* code) { delete code; }
+
+cf_status cf_code_dims_get(const cf_code* h, cf_code_dims* out) {
+    return guarded([&] {
+        if (!h || !out) fail(Kind::Config, "null handle");
+        const NrCode& c = h->code;
+        *out = cf_code_dims{c.k_info, c.e, c.n, c.k, c.m, c.edges_b * c.z, uint32_t(c.bg), c.z, c.ils, c.qm, c.rv};
+    });
+}
+
+cf_status cf_bit_map(const cf_code* h, uint32_t* positions, size_t e) {
+    return guarded([&] {
+        if (!h) fail(Kind::Config, "null handle");
+        expect_len(e, h->code.e, "positions");
+        std::memcpy(positions, h->code.map.data(), e * sizeof(uint32_t));
+    });
+}
+
+cf_status cf_set_decoder(cf_code* h, int rule, double alpha, int layered, uint32_t max_iterations) {
+    return guarded([&] {
+        if (!h) fail(Kind::Config, "null handle");
+        if (rule < 0 || rule > 2)
+            fail(Kind::Config, "rule must be sum_product, min_sum, normalized_min_sum or offset_min_sum, got " +
+                                   std::to_string(rule));
+        std::lock_guard<std::mutex> l(h->mu);
+        h->code.dec = DecoderSettings{rule, alpha, layered != 0, max_iterations};
+    });
+}
+
+cf_status cf_encode_block(const cf_code* h, const uint8_t* info, size_t k_info, uint8_t* out, size_t e) {
+    return guarded([&] {
+        if (!h) fail(Kind::Config, "null handle");
+        expect_len(k_info, h->code.k_info, "bits");
+        expect_len(e, h->code.e, "coded bits");
+        h->code.encode(info, out);
+    });
+}
+
+cf_status cf_decode_llr(const cf_code* h, const double* llr, size_t n, double* post, uint32_t* iterations) {
+    return guarded([&] {
+        if (!h) fail(Kind::Config, "null handle");
+        expect_len(n, h->code.n, "LLRs");
+        decode_host_f64(h, llr, true, nullptr, post, nullptr, iterations);
+    });
+}
+
+cf_status cf_decode_block(const cf_code* h, const double* llr, size_t e, uint8_t* bits, size_t k_info) {
+    return guarded([&] {
+        if (!h) fail(Kind::Config, "null handle");
+        expect_len(e, h->code.e, "LLRs");
+        expect_len(k_info, h->code.k_info, "decoded bits");
+        decode_host_f64(h, llr, false, nullptr, nullptr, bits, nullptr);
+    });
+}
+
+cf_status cf_app_into(const cf_code* h, const double* llr, size_t e, const double* apriori, double* out,
+                      size_t k_info, uint32_t* iterations) {
+    return guarded([&] {
+        if (!h) fail(Kind::Config, "null handle");
+        expect_len(e, h->code.e, "LLRs");
+        expect_len(k_info, h->code.k_info, "a posteriori LLRs");
+        std::vector<double> post(h->code.n);
+        decode_host_f64(h, llr, false, apriori, post.data(), nullptr, iterations);
+        std::memcpy(out, post.data(), k_info * sizeof(double));
+    });
+}
+
+cf_status cf_decode_batch(const cf_code* h, int msg_type, const void* d_llr, size_t count, uint8_t* d_bits,
+                          void* d_post, uint32_t* d_iterations, void* stream) {
+    return guarded([&] {
+        if (!h) fail(Kind::Config, "null handle");
+        if (count == 0) return;
+        if (!d_llr) fail(Kind::Config, "null LLR buffer");
+        int device = current_device();
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        if (msg_type == CF_MSG_F64) {
+            GenericArgs<double> a = generic_args<double>(h, device);
+            a.llr = static_cast<const double*>(d_llr);
+            a.count = int64_t(count);
+            a.bits = d_bits;
+            a.post_out = static_cast<double*>(d_post);
+            a.iterations = d_iterations;
+            run_generic(a, device, s);
+        } else if (msg_type == CF_MSG_F32) {
+            GenericArgs<float> a = generic_args<float>(h, device);
+            a.llr = static_cast<const float*>(d_llr);
+            a.count = int64_t(count);
+            a.bits = d_bits;
+            a.post_out = static_cast<float*>(d_post);
+            a.iterations = d_iterations;
+            run_generic(a, device, s);
+        } else {
+            fail(Kind::Config, "message type must be CF_MSG_F64 or CF_MSG_F32, got " + std::to_string(msg_type));
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,280 @@
+// The general belief-propagation kernel of the lifted NR codes: any base graph and lifting
+// size, layered or flooding schedule, normalized / offset min-sum or sum-product, f32 or
+// f64 messages. It is the decoder of crates/com-fec/src/ldpc.rs:449-557 laid out for a
+// 64-wide wavefront: lane r of a workgroup owns lifted row r of every base row (layer), so
+// one layer is Z independent rows updated in parallel, which is exactly the reference's
+// row-after-row order because the Z rows of a circulant layer share no variable.
+//
+// Per codeword: the posterior LLRs live in LDS when they fit (else in a per-workgroup
+// scratch in HBM); the check-to-variable messages live in a per-workgroup scratch laid out
+// [base edge][r], so lane r's accesses to one edge are contiguous across the wavefront.
+// Workgroups are persistent: each walks codewords blockIdx.x, +gridDim.x, ...
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace comfec {
+
+namespace {
+
+constexpr int kMaxDeg = 19;  // largest row degree of BG1 (BG2: 10)
+
+template <typename T>
+__device__ inline T tanh_t(T x) { return tanh(x); }
+template <typename T>
+__device__ inline T atanh_t(T x) { return atanh(x); }
+
+// ldpc.rs:449-493 for one row of degree `deg`, messages in[] -> out[].
+template <typename T>
+__device__ inline void check_row(int rule, T alpha, const T (&in)[kMaxDeg], T (&out)[kMaxDeg], int deg) {
+    if (rule == 0) {
+        // sum-product: exclusive prefix x suffix products of tanh(l/2), 2 atanh(clamp)
+        T t[kMaxDeg];
+        T prefix = T(1);
+#pragma unroll
+        for (int q = 0; q < kMaxDeg; ++q)
+            if (q < deg) {
+                t[q] = tanh_t(in[q] / T(2));
+                out[q] = prefix;
+                prefix *= t[q];
+            }
+        T suffix = T(1);
+        const T bound = T(0.999999999999);
+#pragma unroll
+        for (int q = kMaxDeg - 1; q >= 0; --q)
+            if (q < deg) {
+                T p = out[q] * suffix;
+                suffix *= t[q];
+                p = p < -bound ? -bound : (p > bound ? bound : p);
+                out[q] = T(2) * atanh_t(p);
+            }
+        return;
+    }
+    // min-sum: sign parity, first minimum (strict <) and second minimum in one scan
+    bool neg = false;
+    T m1 = T(INFINITY), m2 = T(INFINITY);
+    int i1 = 0;
+#pragma unroll
+    for (int q = 0; q < kMaxDeg; ++q)
+        if (q < deg) {
+            T a = fabs(in[q]);
+            neg ^= in[q] < T(0);
+            if (a < m1) {
+                m2 = m1;
+                m1 = a;
+                i1 = q;
+            } else if (a < m2) {
+                m2 = a;
+            }
+        }
+    T g1, g2;
+    if (rule == 1) {
+        g1 = alpha * m1;
+        g2 = alpha * m2;
+    } else {
+        g1 = fmax(m1 - alpha, T(0));
+        g2 = fmax(m2 - alpha, T(0));
+    }
+#pragma unroll
+    for (int q = 0; q < kMaxDeg; ++q)
+        if (q < deg) {
+            T g = q == i1 ? g2 : g1;
+            bool s = neg ^ (in[q] < T(0));
+            out[q] = s ? -g : g;
+        }
+}
+
+template <typename T>
+struct Block {
+    const GenericArgs<T>& a;
+    T* post;
+    T* c2v;
+    T* chan;
+    int tid, nthreads;
+
+    // whether some check fails on the hard decisions of post (ldpc.rs:555-557)
+    __device__ bool syndrome_fails() const {
+        const DevCode& c = a.code;
+        int bad = 0;
+        for (int r = tid; r < c.z; r += nthreads)
+            for (int i = 0; i < c.rows_b && !bad; ++i) {
+                int par = 0;
+                for (int eb = c.row_start[i]; eb < c.row_start[i + 1]; ++eb) {
+                    int rr = r + c.shift[eb];
+                    rr -= rr >= c.z ? c.z : 0;
+                    par ^= post[c.col[eb] * c.z + rr] < T(0);
+                }
+                bad |= par;
+            }
+        return __syncthreads_or(bad) != 0;
+    }
+
+    // ldpc.rs:522-531 for every row of base row i
+    __device__ void layer(int i, bool first) {
+        const DevCode& c = a.code;
+        const int e0 = c.row_start[i], deg = c.row_start[i + 1] - e0;
+        for (int r = tid; r < c.z; r += nthreads) {
+            T in[kMaxDeg], out[kMaxDeg];
+            int idx[kMaxDeg];
+#pragma unroll
+            for (int q = 0; q < kMaxDeg; ++q)
+                if (q < deg) {
+                    int eb = e0 + q;
+                    int rr = r + c.shift[eb];
+                    rr -= rr >= c.z ? c.z : 0;
+                    idx[q] = c.col[eb] * c.z + rr;
+                    T old = first ? T(0) : c2v[size_t(eb) * c.z + r];
+                    in[q] = post[idx[q]] - old;
+                }
+            check_row<T>(a.rule, T(a.alpha), in, out, deg);
+#pragma unroll
+            for (int q = 0; q < kMaxDeg; ++q)
+                if (q < deg) {
+                    post[idx[q]] = in[q] + out[q];
+                    c2v[size_t(e0 + q) * c.z + r] = out[q];
+                }
+        }
+    }
+
+    // ldpc.rs:533-544: new messages of every row from the old posteriors, in place
+    __device__ void flood_checks(bool first) {
+        const DevCode& c = a.code;
+        for (int i = 0; i < c.rows_b; ++i) {
+            const int e0 = c.row_start[i], deg = c.row_start[i + 1] - e0;
+            for (int r = tid; r < c.z; r += nthreads) {
+                T in[kMaxDeg], out[kMaxDeg];
+#pragma unroll
+                for (int q = 0; q < kMaxDeg; ++q)
+                    if (q < deg) {
+                        int eb = e0 + q;
+                        int rr = r + c.shift[eb];
+                        rr -= rr >= c.z ? c.z : 0;
+                        T old = first ? T(0) : c2v[size_t(eb) * c.z + r];
+                        in[q] = post[c.col[eb] * c.z + rr] - old;
+                    }
+                check_row<T>(a.rule, T(a.alpha), in, out, deg);
+#pragma unroll
+                for (int q = 0; q < kMaxDeg; ++q)
+                    if (q < deg) c2v[size_t(e0 + q) * c.z + r] = out[q];
+            }
+        }
+    }
+
+    // post = channel + the new messages of every check, in ascending check order
+    __device__ void flood_variables() {
+        const DevCode& c = a.code;
+        for (int p = tid; p < c.n; p += nthreads) {
+            int j = p / c.z, q = p - j * c.z;
+            T v = chan[p];
+            for (int s = c.col_start[j]; s < c.col_start[j + 1]; ++s) {
+                int eb = c.col_edge[s];
+                int r = q - int(c.shift[eb]);
+                r += r < 0 ? c.z : 0;
+                v += c2v[size_t(eb) * c.z + r];
+            }
+            post[p] = v;
+        }
+    }
+};
+
+template <typename T, bool kPostLds>
+__global__ void __launch_bounds__(512) decode_generic_kernel(GenericArgs<T> a) {
+    extern __shared__ unsigned char lds_raw[];
+    const DevCode& c = a.code;
+    T* scratch = a.scratch + size_t(blockIdx.x) * a.scratch_stride;
+    Block<T> b{a, nullptr, scratch, nullptr, int(threadIdx.x), int(blockDim.x)};
+    size_t off = size_t(c.edges_b) * c.z;
+    if (kPostLds) {
+        b.post = reinterpret_cast<T*>(lds_raw);
+    } else {
+        b.post = scratch + off;
+        off += c.n;
+    }
+    b.chan = a.layered ? nullptr : scratch + off;
+
+    for (int64_t w = blockIdx.x; w < a.count; w += gridDim.x) {
+        // the channel LLRs of the full codeword (nr/ldpc.rs:191-197 or ldpc.rs:500-505)
+        for (int p = b.tid; p < c.n; p += b.nthreads) {
+            T v;
+            if (a.input_full) {
+                v = a.llr[w * c.n + p];
+            } else {
+                v = T(0);
+                for (uint32_t s = c.src_start[p]; s < c.src_start[p + 1]; ++s) v += a.llr[w * c.e + c.src[s]];
+                if (p >= c.k_info && p < c.k) v = T(30);
+            }
+            if (a.apriori && p < c.k_info) v += a.apriori[w * c.k_info + p];
+            b.post[p] = v;
+            if (!a.layered) b.chan[p] = v;
+        }
+        __syncthreads();
+        int done = b.syndrome_fails() ? -1 : 0;
+        for (int it = 1; it <= a.max_iterations && done < 0; ++it) {
+            if (a.layered) {
+                for (int i = 0; i < c.rows_b; ++i) {
+                    b.layer(i, it == 1);
+                    __syncthreads();
+                }
+            } else {
+                b.flood_checks(it == 1);
+                __syncthreads();
+                b.flood_variables();
+                __syncthreads();
+            }
+            if (!b.syndrome_fails()) done = it;
+        }
+        if (done < 0) done = a.max_iterations;
+        if (b.tid == 0 && a.iterations) a.iterations[w] = uint32_t(done);
+        if (a.bits)
+            for (int p = b.tid; p < c.k_info; p += b.nthreads) a.bits[w * c.k_info + p] = b.post[p] < T(0);
+        if (a.post_out)
+            for (int p = b.tid; p < c.n; p += b.nthreads) a.post_out[w * c.n + p] = b.post[p];
+        __syncthreads();
+    }
+}
+
+template <typename T>
+hipError_t launch(const GenericArgs<T>& a0, int device, hipStream_t stream, Workspace& ws) {
+    GenericArgs<T> a = a0;
+    const DevCode& c = a.code;
+    int threads = ((c.z + 63) / 64) * 64;
+    if (threads > 512) threads = 512;
+    const size_t post_bytes = size_t(c.n) * sizeof(T);
+    const bool post_lds = post_bytes <= kLdsBudget;
+    hipDeviceProp_t prop;
+    hipError_t err = hipGetDeviceProperties(&prop, device);
+    if (err != hipSuccess) return err;
+    int per_cu = 0;
+    if (post_lds)
+        err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, decode_generic_kernel<T, true>, threads,
+                                                           post_bytes);
+    else
+        err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, decode_generic_kernel<T, false>, threads, 0);
+    if (err != hipSuccess) return err;
+    if (per_cu < 1) per_cu = 1;
+    int64_t grid = int64_t(per_cu) * prop.multiProcessorCount;
+    if (grid > a.count) grid = a.count;
+    if (grid < 1) return hipSuccess;
+    a.scratch_stride = size_t(c.edges_b) * c.z + (post_lds ? 0 : c.n) + (a.layered ? 0 : c.n);
+    a.scratch_stride = (a.scratch_stride + 63) & ~size_t(63);
+    T* scratch = static_cast<T*>(ws.get(size_t(grid) * a.scratch_stride * sizeof(T)));
+    if (!scratch) return hipErrorOutOfMemory;
+    a.scratch = scratch;
+    if (post_lds)
+        hipLaunchKernelGGL((decode_generic_kernel<T, true>), dim3(unsigned(grid)), dim3(threads), post_bytes, stream,
+                           a);
+    else
+        hipLaunchKernelGGL((decode_generic_kernel<T, false>), dim3(unsigned(grid)), dim3(threads), 0, stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_generic(const GenericArgs<float>& a, int device, hipStream_t s, Workspace& ws) {
+    return launch<float>(a, device, s, ws);
+}
+hipError_t launch_generic(const GenericArgs<double>& a, int device, hipStream_t s, Workspace& ws) {
+    return launch<double>(a, device, s, ws);
+}
+
+}  // namespace comfec

@@ -1,0 +1,120 @@
+/*
+ * comfec_gpu: the NR LDPC belief-propagation decoder of com-fec and the Monte Carlo trial
+ * loop of com-sim, on one AMD Instinct MI355X (gfx950), behind a C ABI.
+ *
+ * Every entry point names the reference interface it replaces (paths relative to the
+ * com-rs repository). Conventions, as in the reference:
+ *   - LLRs are positive for a 0 (crates/com-core/src/signal.rs:1-49); a hard decision is
+ *     1 for an LLR < 0.0 (so -0.0 decides 0).
+ *   - Errors are the kinds of com_core::Error (crates/com-core/src/error.rs:25-41), plus
+ *     CF_DEVICE for a HIP failure; cf_last_error() gives the message of the last failure
+ *     of the calling thread, worded as the reference's ("inconsistent shapes: 10 LLRs
+ *     for a block of 300").
+ *   - The caller owns every buffer; the library owns the cf_code handles. A handle is
+ *     immutable during a decode (the settings change only through cf_set_decoder), and
+ *     calls on one handle from several threads are safe.
+ *   - "_host" buffers are host memory; "d_" buffers are device memory of the current
+ *     HIP device; `stream` is a hipStream_t (NULL: the default stream).
+ */
+#ifndef COMFEC_GPU_H
+#define COMFEC_GPU_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cf_code cf_code;
+
+typedef enum {
+    CF_OK = 0,
+    CF_CONFIG = 1,    /* Error::Config */
+    CF_SHAPE = 2,     /* Error::Shape */
+    CF_NUMERICAL = 3, /* Error::Numerical */
+    CF_CONDITION = 4, /* Error::Condition */
+    CF_IO = 5,        /* Error::Io */
+    CF_DEVICE = 6     /* a HIP runtime failure (no reference counterpart) */
+} cf_status;
+
+/* CheckRule (crates/com-fec/src/ldpc.rs:95-104) */
+enum { CF_RULE_SUM_PRODUCT = 0, CF_RULE_NORMALIZED_MIN_SUM = 1, CF_RULE_OFFSET_MIN_SUM = 2 };
+
+/* Message arithmetic of the device decoder. F64 reproduces the reference's f64 decoder
+   bit for bit (min-sum rules); F32 keeps FP32 messages; F16 stores the check-to-variable
+   messages in FP16 and computes in FP32. */
+enum { CF_MSG_F64 = 0, CF_MSG_F32 = 1, CF_MSG_F16 = 2 };
+
+typedef struct {
+    uint32_t k_info;    /* K': information bits (BlockCode::k) */
+    uint32_t e;         /* E: transmitted bits (BlockCode::n) */
+    uint32_t n;         /* N: bits of the lifted code (LdpcCode::n) */
+    uint32_t k;         /* K: systematic columns, 22Z or 10Z (LdpcCode::k) */
+    uint32_t m;         /* checks */
+    uint32_t edges;     /* nonzero entries of H */
+    uint32_t base_graph;
+    uint32_t z;
+    uint32_t ils;       /* lifting-set index */
+    uint32_t qm;
+    uint32_t rv;
+} cf_code_dims;
+
+/* The message of the last failed call of this thread ("" after a success). */
+const char* cf_last_error(void);
+
+/* NrLdpc::with_options (crates/com-fec/src/nr/ldpc.rs:101-162): base_graph 0 chooses it as
+   Sec. 7.2.2 does (nr/ldpc.rs:78-81). Settings are LdpcDecoder::default() (ldpc.rs:171-175). */
+cf_status cf_nr_create(uint32_t k_info, uint32_t e, int base_graph, int rv, uint32_t qm,
+                       cf_code** out);
+
+/* LdpcCode::from_core_rows (ldpc.rs:270-287) on the lifted base graph bg at lifting size z
+   (the construction of ldpc.rs:934-948): K' = K = 22Z or 10Z, the N - 2Z columns from 2Z
+   transmitted once, in order. */
+cf_status cf_lifted_create(int base_graph, uint32_t z, cf_code** out);
+
+void cf_code_destroy(cf_code* code);
+
+/* NrLdpc::{k, n, base_graph, lifting_size} and LdpcCode::{n, k} (nr/ldpc.rs:164-187). */
+cf_status cf_code_dims_get(const cf_code* code, cf_code_dims* out);
+
+/* NrLdpc::bit_map (nr/ldpc.rs:175-177): the full-codeword bit of every transmitted bit. */
+cf_status cf_bit_map(const cf_code* code, uint32_t* positions_host, size_t e);
+
+/* The public fields of LdpcDecoder (ldpc.rs:164-179) through NrLdpc::decoder_mut
+   (nr/ldpc.rs:184-187): rule, its alpha (scale of the normalized min-sum, offset of the
+   offset min-sum), layered or flooding, maximum iterations (0: channel LLRs returned). */
+cf_status cf_set_decoder(cf_code* code, int rule, double alpha, int layered,
+                         uint32_t max_iterations);
+
+/* BlockCode::encode_block (nr/ldpc.rs:208-214): K' bits in, E bits out (host). */
+cf_status cf_encode_block(const cf_code* code, const uint8_t* info_host, size_t k_info,
+                          uint8_t* out_host, size_t e);
+
+/* LdpcCode::decode_llr (ldpc.rs:500-505): N channel LLRs in, N a posteriori LLRs and the
+   iterations out. Runs on the GPU in f64, bit-identical to the reference for min-sum. */
+cf_status cf_decode_llr(const cf_code* code, const double* llr_host, size_t n,
+                        double* post_host, uint32_t* iterations);
+
+/* BlockCode::decode_block (nr/ldpc.rs:216-220): E LLRs in, K' hard decisions out. */
+cf_status cf_decode_block(const cf_code* code, const double* llr_host, size_t e,
+                          uint8_t* bits_host, size_t k_info);
+
+/* SoftDecode::app_into (nr/ldpc.rs:226-239): E LLRs and optional K' a priori LLRs in,
+   K' a posteriori LLRs out; returns the iterations through *iterations. */
+cf_status cf_app_into(const cf_code* code, const double* llr_host, size_t e,
+                      const double* apriori_host, double* out_host, size_t k_info,
+                      uint32_t* iterations);
+
+/* BlockCode::decode_stream (crates/com-fec/src/block.rs:81-89) and the trial loop of
+   crates/com-sim/src/runner.rs:435-446 as one launch: `count` codewords of the same code,
+   E LLRs each (f32 for CF_MSG_F32/F16, f64 for CF_MSG_F64) resident in device memory.
+   Outputs (device, any may be NULL): d_bits count x K' hard decisions; d_post count x N a
+   posteriori LLRs (of the input type); d_iterations count iteration counts (0 when the
+   input was a codeword). Asynchronous on `stream`. */
+cf_status cf_decode_batch(const cf_code* code, int msg_type, const void* d_llr, size_t count,
+                          uint8_t* d_bits, void* d_post, uint32_t* d_iterations, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,148 @@
+"""Parity of the HIP decoder with the oracle, through the C ABI.
+
+- f64 messages: bit-identical posteriors and iteration counts for the min-sum rules
+  (the same IEEE operations in the same order, no FMA contraction).
+- f32 messages: identical hard decisions, frame and bit error counts and iteration
+  counts, posteriors within FP32 rounding of the f64 values (tolerances written below),
+  except on recorded ties (|posterior| below TIE).
+"""
+import numpy as np
+import pytest
+
+import comfec as F
+import oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+SEED = 1
+TIE = 1e-6
+
+
+def cuda(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def config1_llrs(count, ebn0_db=1.0, name="cfg1"):
+    """BG2 Z=52 lifted, K=520, BPSK over AWGN, LLRs 2y/s2 of the 2600 sent bits."""
+    ref = O.Code.lifted(2, 52)
+    cws, llrs = [], []
+    for t in range(count):
+        cw, llr = ref.trial_bpsk_lifted(ebn0_db, SEED, name, 0, t)
+        cws.append(cw)
+        llrs.append(llr)
+    return ref, np.array(cws), np.array(llrs)
+
+
+RULES = [("nms", O.NMS, "normalized_min_sum", 0.75), ("oms", O.OMS, "offset_min_sum", 0.15)]
+
+
+@pytest.mark.parametrize("layered", [True, False], ids=["layered", "flooding"])
+@pytest.mark.parametrize("rule", RULES, ids=[r[0] for r in RULES])
+def test_f64_batch_is_bit_exact(rule, layered):
+    print(f"\n[f64 batch] rule={rule[0]} layered={layered}", flush=True)
+    _, orule, name, alpha = rule
+    ref, cws, full = config1_llrs(48)
+    post_ref, it_ref = ref.decode_many(full, rule=orule, alpha=alpha, layered=layered, max_iterations=20,
+                                       threads=8)
+    code = F.NrLdpc.lifted(2, 52).set_decoder(name, alpha, layered, 20)
+    bits, its, post = code.decode_batch(cuda(full[:, 104:]), want_post=True)
+    torch.cuda.synchronize()
+    post = post.cpu().numpy()
+    assert np.array_equal(its.cpu().numpy(), it_ref)
+    assert np.array_equal(post, post_ref)
+    assert np.array_equal(bits.cpu().numpy(), (post_ref[:, :520] < 0).astype(np.uint8))
+    # the workload is not trivial: the frames stop at many different iterations
+    assert len(set(it_ref.tolist())) >= 4
+
+
+@pytest.mark.parametrize("k,e,rv,qm", [(100, 300, 0, 1), (500, 1200, 2, 4), (60, 1000, 1, 2), (1000, 1500, 3, 6)])
+def test_host_entry_points_match_oracle(k, e, rv, qm):
+    print(f"\n[host f64] k={k} e={e} rv={rv} qm={qm}", flush=True)
+    code = F.NrLdpc(k, e, None, rv, qm).set_decoder("normalized_min_sum", 0.75, True, 25)
+    ref = O.Code.nr(k, e, 0, rv, qm)
+    rng = O.Rng.stream(SEED, "host", k, e)
+    info = np.array([rng.bit() for _ in range(k)], np.uint8)
+    tx = ref.nr_encode(info)
+    s2 = O.lib.or_bpsk_noise_variance(1.5, k / e)
+    llr = np.zeros(e)
+    O.lib.or_bpsk_awgn_llr(tx, e, s2, rng.r, llr)
+    bits_ref, it_ref = ref.decode_block(llr, rule=O.NMS, alpha=0.75, max_iterations=25)
+    assert np.array_equal(code.decode_block(llr), bits_ref)
+    full = ref.full_llr(llr)
+    post_ref, it2 = ref.decode_llr(full, rule=O.NMS, alpha=0.75, max_iterations=25)
+    post, it = code.decode_llr(full)
+    assert it == it2 == it_ref and np.array_equal(post, post_ref)
+    apri = np.linspace(-0.5, 0.5, k)
+    app, it3 = code.app(llr, apri)
+    full2 = full.copy()
+    full2[:k] += apri
+    post3, it3_ref = ref.decode_llr(full2, rule=O.NMS, alpha=0.75, max_iterations=25)
+    assert it3 == it3_ref and np.array_equal(app, post3[:k])
+
+
+def test_codeword_input_and_zero_iterations():
+    print("\n[edge] codeword input, max_iterations 0, empty batch", flush=True)
+    code = F.NrLdpc.lifted(2, 52).set_decoder("normalized_min_sum", 0.75, True, 20)
+    ref = O.Code.lifted(2, 52)
+    cw = ref.encode(np.random.default_rng(3).integers(0, 2, ref.k).astype(np.uint8))
+    llr = np.where(cw == 0, 2.0, -2.0)
+    post, it = code.decode_llr(llr)
+    assert it == 0 and np.array_equal(post, llr)
+    noisy = llr + np.random.default_rng(4).normal(0, 2.5, llr.size)
+    code.set_decoder("normalized_min_sum", 0.75, True, 0)
+    post, it = code.decode_llr(noisy)
+    assert it == 0 and np.array_equal(post, noisy)
+    bits, its, _ = code.decode_batch(cuda(np.zeros((0, code.n))))
+    assert bits.shape == (0, 520)
+    # -0.0 is a 0 decision, as in the reference (x < 0.0)
+    z = np.zeros(ref.n)
+    z[:] = -0.0
+    code.set_decoder("normalized_min_sum", 0.75, True, 5)
+    post, it = code.decode_llr(z)
+    assert it == 0
+
+
+def test_f64_sum_product_within_libm_rounding():
+    print("\n[f64 spa]", flush=True)
+    ref, cws, full = config1_llrs(16, ebn0_db=1.5, name="spa")
+    post_ref, it_ref = ref.decode_many(full, rule=O.SPA, layered=True, max_iterations=20, threads=8)
+    code = F.NrLdpc.lifted(2, 52).set_decoder("sum_product", 0.0, True, 20)
+    bits, its, post = code.decode_batch(cuda(full[:, 104:]), want_post=True)
+    post = post.cpu().numpy()
+    its = its.cpu().numpy()
+    same = its == it_ref
+    assert same.mean() >= 0.9
+    # device tanh/atanh differ from glibc's in the last ulps
+    np.testing.assert_allclose(post[same], post_ref[same], rtol=1e-9, atol=1e-9)
+
+
+def config2_llrs(esn0_db, count, k=8448, e=25344, qm=4, m=16):
+    ref = O.Code.nr(k, e, 1, 0, qm)
+    n0 = 1.0 / O.lib.or_db2lin(esn0_db)
+    infos, llrs = [], []
+    for t in range(count):
+        info, llr = ref.trial_nr_qam(m, n0, SEED, "cfg2", int(esn0_db), t)
+        infos.append(info)
+        llrs.append(llr)
+    return ref, np.array(infos), np.array(llrs)
+
+
+@pytest.mark.parametrize("esn0_db", [4.0, 5.0, 6.0, 7.0])
+def test_f32_bg1_z384_matches_f64_reference(esn0_db):
+    print(f"\n[f32 BG1 Z=384 16-QAM] Es/N0={esn0_db}", flush=True)
+    count = 12
+    ref, infos, llrs = config2_llrs(esn0_db, count)
+    fulls = np.array([ref.full_llr(l) for l in llrs])
+    post_ref, it_ref = ref.decode_many(fulls, rule=O.NMS, alpha=0.75, layered=True, max_iterations=10, threads=8)
+    code = F.NrLdpc(8448, 25344, 1, 0, 4).set_decoder("normalized_min_sum", 0.75, True, 10)
+    bits, its, post = code.decode_batch(cuda(llrs.astype(np.float32)), want_post=True)
+    bits, its, post = bits.cpu().numpy(), its.cpu().numpy(), post.cpu().numpy().astype(np.float64)
+    bits_ref = (post_ref[:, :8448] < 0).astype(np.uint8)
+    ties = np.abs(post_ref[:, :8448]) < TIE
+    assert np.array_equal(bits[~ties], bits_ref[~ties])
+    assert np.array_equal((bits != infos).sum(1), (bits_ref != infos).sum(1))
+    assert np.array_equal(its, it_ref)
+    # FP32 rounding of messages that are sums of up to ~30 terms over <= 10 iterations
+    scale = np.maximum(np.abs(post_ref), 1.0)
+    assert np.max(np.abs(post - post_ref) / scale) < 1e-4
