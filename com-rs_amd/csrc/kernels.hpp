@@ -40,6 +40,27 @@ struct GenericArgs {
     size_t scratch_stride;    // elements per workgroup
 };
 
+// The fast layered min-sum kernel (FP32 messages, check-to-variable messages in registers).
+struct LayeredArgs {
+    int32_t z, k, k_info, e;
+    const uint16_t* row_start;  // base tables (device)
+    const uint16_t* col;
+    const uint16_t* shift;
+    const uint32_t* src_start;
+    const uint32_t* src;
+    float a, b;                 // g(m) = a * max(m - b, 0)
+    const float* params;        // learned: (a, b) per iteration and layer, or null
+    int max_iterations;
+    const float* llr;
+    int64_t count;
+    uint8_t* bits;
+    float* post_out;
+    uint32_t* iterations;
+};
+
+bool layered_supported(int bg, int z);
+hipError_t launch_layered(const LayeredArgs& a, int bg, hipStream_t stream);
+
 // A grow-only device buffer, one per (device, stream) in the C ABI layer.
 struct Workspace {
     void* ptr = nullptr;

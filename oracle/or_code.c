@@ -279,119 +279,35 @@ int or_nr_full_llr(const or_code* c, const double* llr, double* full) {
     return 0;
 }
 
-/* ---- decoder ------------------------------------------------------------------------ */
+/* ---- decoder: the same body in f64 (the reference) and in f32 ------------------------ */
 
-#define TANH_BOUND 0.999999999999
+#define REAL double
+#define SFX(name) name
+#define FABS fabs
+#define FMAX fmax
+#define TANH tanh
+#define ATANH atanh
+#include "or_bp_body.h"
+#undef REAL
+#undef SFX
+#undef FABS
+#undef FMAX
+#undef TANH
+#undef ATANH
 
-/* ldpc.rs:449-493 */
-static void check_message(const or_dec* d, const double* in, int deg, double* out, double* scratch) {
-    if (d->rule == OR_SPA) {
-        for (int i = 0; i < deg; ++i) scratch[i] = tanh(in[i] / 2.0);
-        double prefix = 1.0;
-        for (int i = 0; i < deg; ++i) {
-            out[i] = prefix;
-            prefix *= scratch[i];
-        }
-        double suffix = 1.0;
-        for (int i = deg - 1; i >= 0; --i) {
-            double p = out[i] * suffix;
-            suffix *= scratch[i];
-            if (p < -TANH_BOUND) p = -TANH_BOUND;
-            if (p > TANH_BOUND) p = TANH_BOUND;
-            out[i] = 2.0 * atanh(p);
-        }
-        return;
-    }
-    double sign = 1.0;
-    for (int i = 0; i < deg; ++i) sign *= in[i] < 0.0 ? -1.0 : 1.0;
-    double m1 = INFINITY, m2 = INFINITY;
-    int i1 = 0;
-    for (int i = 0; i < deg; ++i) {
-        double a = fabs(in[i]);
-        if (a < m1) {
-            m2 = m1;
-            m1 = a;
-            i1 = i;
-        } else if (a < m2) {
-            m2 = a;
-        }
-    }
-    for (int i = 0; i < deg; ++i) {
-        double mag = i == i1 ? m2 : m1;
-        double s = sign * (in[i] < 0.0 ? -1.0 : 1.0);
-        if (d->rule == OR_NMS) mag = d->alpha * mag;
-        else mag = fmax(mag - d->alpha, 0.0);
-        out[i] = s * mag;
-    }
-}
-
-/* ldpc.rs:555-557 */
-static int is_codeword_post(const or_code* c, const double* post) {
-    for (int32_t r = 0; r < c->m; ++r) {
-        uint8_t a = 0;
-        for (int32_t e = c->row_start[r]; e < c->row_start[r + 1]; ++e) a ^= (uint8_t)(post[c->cols[e]] < 0.0);
-        if (a) return 0;
-    }
-    return 1;
-}
-
-static int max_degree(const or_code* c) {
-    int dm = 0;
-    for (int32_t r = 0; r < c->m; ++r) {
-        int d = c->row_start[r + 1] - c->row_start[r];
-        if (d > dm) dm = d;
-    }
-    return dm;
-}
-
-/* ldpc.rs:509-551 */
-static int propagate(const or_code* c, const or_dec* d, const double* channel, double* post) {
-    if (is_codeword_post(c, post)) return 0;
-    int32_t edges = c->row_start[c->m];
-    double* c2v = (double*)calloc((size_t)edges, sizeof(double));
-    double* new_c2v = d->layered ? NULL : (double*)calloc((size_t)edges, sizeof(double));
-    int dm = max_degree(c);
-    double* incoming = (double*)malloc(sizeof(double) * (dm + 1));
-    double* scratch = (double*)malloc(sizeof(double) * (dm + 1));
-    int result = d->max_iterations;
-    for (int it = 1; it <= d->max_iterations; ++it) {
-        if (d->layered) {
-            for (int32_t r = 0; r < c->m; ++r) {
-                int32_t s = c->row_start[r];
-                int deg = c->row_start[r + 1] - s;
-                double* msgs = c2v + s;
-                for (int i = 0; i < deg; ++i) incoming[i] = post[c->cols[s + i]] - msgs[i];
-                check_message(d, incoming, deg, msgs, scratch);
-                for (int i = 0; i < deg; ++i) post[c->cols[s + i]] = incoming[i] + msgs[i];
-            }
-        } else {
-            for (int32_t r = 0; r < c->m; ++r) {
-                int32_t s = c->row_start[r];
-                int deg = c->row_start[r + 1] - s;
-                for (int i = 0; i < deg; ++i) incoming[i] = post[c->cols[s + i]] - c2v[s + i];
-                check_message(d, incoming, deg, new_c2v + s, scratch);
-            }
-            memcpy(post, channel, sizeof(double) * c->n);
-            for (int32_t r = 0; r < c->m; ++r)
-                for (int32_t e = c->row_start[r]; e < c->row_start[r + 1]; ++e) post[c->cols[e]] += new_c2v[e];
-            double* t = c2v;
-            c2v = new_c2v;
-            new_c2v = t;
-        }
-        if (is_codeword_post(c, post)) { result = it; break; }
-    }
-    free(c2v);
-    free(new_c2v);
-    free(incoming);
-    free(scratch);
-    return result;
-}
-
-/* ldpc.rs:500-505 */
-int or_decode_llr(const or_code* c, const or_dec* d, const double* llr, double* post) {
-    memcpy(post, llr, sizeof(double) * c->n);
-    return propagate(c, d, llr, post);
-}
+#define REAL float
+#define SFX(name) name##_f32
+#define FABS fabsf
+#define FMAX fmaxf
+#define TANH tanhf
+#define ATANH atanhf
+#include "or_bp_body.h"
+#undef REAL
+#undef SFX
+#undef FABS
+#undef FMAX
+#undef TANH
+#undef ATANH
 
 int or_nr_decode_block(const or_code* c, const or_dec* d, const double* llr, uint8_t* bits) {
     double* full = (double*)malloc(sizeof(double) * c->n);
@@ -402,38 +318,6 @@ int or_nr_decode_block(const or_code* c, const or_dec* d, const double* llr, uin
     free(full);
     free(post);
     return it;
-}
-
-typedef struct {
-    const or_code* c;
-    const or_dec* d;
-    const double* llr;
-    double* post;
-    int32_t* its;
-    int64_t begin, end;
-} many_job;
-
-static void* many_worker(void* arg) {
-    many_job* j = (many_job*)arg;
-    int32_t n = j->c->n;
-    for (int64_t w = j->begin; w < j->end; ++w)
-        j->its[w] = or_decode_llr(j->c, j->d, j->llr + w * n, j->post + w * n);
-    return NULL;
-}
-
-void or_decode_many(const or_code* c, const or_dec* d, const double* llr, int64_t count,
-                    double* post, int32_t* iterations, int threads) {
-    if (threads < 1) threads = 1;
-    if (threads > 256) threads = 256;
-    pthread_t th[256];
-    many_job jobs[256];
-    for (int t = 0; t < threads; ++t) {
-        jobs[t] = (many_job){c, d, llr, post, iterations, count * t / threads, count * (t + 1) / threads};
-        if (threads == 1) many_worker(&jobs[t]);
-        else pthread_create(&th[t], NULL, many_worker, &jobs[t]);
-    }
-    if (threads > 1)
-        for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
 }
 
 /* ---- trial chains -------------------------------------------------------------------- */

@@ -64,6 +64,12 @@ int or_nr_decode_block(const or_code* c, const or_dec* d, const double* llr, uin
 void or_decode_many(const or_code* c, const or_dec* d, const double* llr, int64_t count,
                     double* post, int32_t* iterations, int threads);
 
+/* the same decoder with every message rounded to FP32 (min-sum rules: the exact target of
+   the FP32 device kernels, whose operations are the same IEEE FP32 operations) */
+int or_decode_llr_f32(const or_code* c, const or_dec* d, const float* llr, float* post);
+void or_decode_many_f32(const or_code* c, const or_dec* d, const float* llr, int64_t count,
+                        float* post, int32_t* iterations, int threads);
+
 /* learned offset/scale min-sum (no reference counterpart): layered schedule, iteration t
    and base row i use g(m) = alpha[t][i] * max(m - beta[t][i], 0); params t-major. */
 int or_learned_decode(const or_code* c, int iterations, const double* alpha, const double* beta,

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 _u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 
 
 class OrRng(C.Structure):
@@ -63,6 +64,8 @@ def _load():
         "or_decode_llr": (C.c_int, [C.c_void_p, C.POINTER(OrDec), _f64p, _f64p]),
         "or_nr_decode_block": (C.c_int, [C.c_void_p, C.POINTER(OrDec), _f64p, _u8p]),
         "or_decode_many": (None, [C.c_void_p, C.POINTER(OrDec), _f64p, C.c_int64, _f64p, _i32p, C.c_int]),
+        "or_decode_llr_f32": (C.c_int, [C.c_void_p, C.POINTER(OrDec), _f32p, _f32p]),
+        "or_decode_many_f32": (None, [C.c_void_p, C.POINTER(OrDec), _f32p, C.c_int64, _f32p, _i32p, C.c_int]),
         "or_db2lin": (C.c_double, [C.c_double]),
         "or_bpsk_noise_variance": (C.c_double, [C.c_double, C.c_double]),
         "or_ebn0_noise_variance": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_double]),
@@ -200,6 +203,15 @@ class Code:
         post = np.zeros_like(llr)
         its = np.zeros(len(llr), np.int32)
         lib.or_decode_many(self.h, C.byref(d), llr, len(llr), post, its, threads)
+        return post, its
+
+    def decode_many_f32(self, llr, rule=NMS, alpha=0.75, layered=True, max_iterations=20, threads=1):
+        """The decoder with FP32 messages (the same operations rounded to FP32)."""
+        llr = np.ascontiguousarray(llr, np.float32).reshape(-1, self.n)
+        d = OrDec(rule, alpha, int(layered), max_iterations)
+        post = np.zeros_like(llr)
+        its = np.zeros(len(llr), np.int32)
+        lib.or_decode_many_f32(self.h, C.byref(d), llr, len(llr), post, its, threads)
         return post, its
 
     def trial_bpsk_lifted(self, ebn0_db, seed, name, cond, trial):
