@@ -54,6 +54,51 @@ SIGNATURES = [
                                   C.c_void_p, C.c_void_p]),
 ]
 
+class StopNode(C.Structure):
+    _fields_ = [("kind", C.c_int), ("n_children", C.c_uint32), ("count", C.c_uint64), ("value", C.c_double),
+                ("confidence", C.c_double), ("metric", C.c_char_p)]
+
+
+class GridAxis(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("values", C.POINTER(C.c_double)), ("n_values", C.c_size_t)]
+
+
+class RunnerConfig(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("batch", C.c_uint64), ("confidence", C.c_double), ("fast_dev_run", C.c_int)]
+
+
+class NrLink(C.Structure):
+    _fields_ = [("modulation", C.c_int), ("snr_kind", C.c_int), ("d_learned_params", C.c_void_p)]
+
+
+class Metric(C.Structure):
+    _fields_ = [("value", C.c_double), ("lo", C.c_double), ("hi", C.c_double), ("count", C.c_uint64),
+                ("total", C.c_uint64), ("is_rate", C.c_int), ("dropped", C.c_uint64)]
+
+
+BATCH_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_double), C.c_size_t, C.c_uint64,
+                       C.c_uint64, C.c_void_p)
+
+SIGNATURES += [
+    ("cf_generate_batch", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_uint32, C.c_uint64,
+                                    C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("cf_record_errors", C.c_int, [C.c_void_p, C.c_char_p, C.c_uint64, C.c_uint64]),
+    ("cf_record_value", C.c_int, [C.c_void_p, C.c_char_p, C.c_double]),
+    ("cf_record_fail", C.c_int, [C.c_void_p, C.c_int, C.c_char_p]),
+    ("cf_run", C.c_int, [C.POINTER(RunnerConfig), C.POINTER(StopNode), C.c_size_t, C.POINTER(GridAxis), C.c_size_t,
+                         C.c_char_p, BATCH_FN, C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("cf_nr_campaign", C.c_int, [C.c_void_p, C.POINTER(NrLink), C.POINTER(RunnerConfig), C.POINTER(StopNode),
+                                 C.c_size_t, C.POINTER(GridAxis), C.c_size_t, C.POINTER(C.c_void_p)]),
+    ("cf_results_count", C.c_size_t, [C.c_void_p]),
+    ("cf_results_condition", C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_double)]),
+    ("cf_results_metric", C.c_int, [C.c_void_p, C.c_size_t, C.c_char_p, C.c_double, C.POINTER(Metric)]),
+    ("cf_results_free", None, [C.c_void_p]),
+]
+
+BPSK, QPSK, QAM16, QAM64 = 1, 2, 4, 6
+EBN0_DB, ESN0_DB = 0, 1
+
 _lib = None
 
 
@@ -207,3 +252,219 @@ class NrLdpc:
                                          C.c_void_p(post.data_ptr()) if post is not None else None,
                                          C.c_void_p(its.data_ptr()), _stream_ptr(stream)))
         return bits, its, post
+
+
+# ---- campaigns: the Runner / Stop / Grid API of crates/com-sim -------------------------------
+
+STOP_MAX_TRIALS, STOP_MIN_TRIALS, STOP_MIN_ERRORS, STOP_PRECISION, STOP_MAX_TIME, STOP_ANY, STOP_ALL = range(7)
+
+
+class Stop:
+    """stopping.rs:64-141: the rules and their `or` / `and` combinations."""
+
+    def __init__(self, kind, count=0, value=0.0, confidence=0.95, metric=None, children=()):
+        self.kind, self.count, self.value, self.confidence, self.metric = kind, count, value, confidence, metric
+        self.children = list(children)
+
+    @staticmethod
+    def max_trials(n):
+        return Stop(STOP_MAX_TRIALS, count=n)
+
+    @staticmethod
+    def min_trials(n):
+        return Stop(STOP_MIN_TRIALS, count=n)
+
+    @staticmethod
+    def min_errors(metric, errors):
+        return Stop(STOP_MIN_ERRORS, count=errors, metric=metric)
+
+    @staticmethod
+    def precision(metric, relative, confidence=0.95):
+        return Stop(STOP_PRECISION, value=relative, confidence=confidence, metric=metric)
+
+    @staticmethod
+    def max_time(seconds):
+        return Stop(STOP_MAX_TIME, value=seconds)
+
+    def or_(self, other):
+        return Stop(STOP_ANY, children=[self, other])
+
+    def and_(self, other):
+        return Stop(STOP_ALL, children=[self, other])
+
+    __or__ = or_
+    __and__ = and_
+
+    def nodes(self):
+        out = [StopNode(self.kind, len(self.children), int(self.count), float(self.value), float(self.confidence),
+                        self.metric.encode() if self.metric else None)]
+        for c in self.children:
+            out.extend(c.nodes())
+        return out
+
+
+class Grid:
+    """grid.rs:231-290: axes in order (the last fastest) and fixed values."""
+
+    def __init__(self):
+        self.parts = []
+
+    def axis(self, name, values):
+        self.parts.append((name, [float(v) for v in values]))
+        return self
+
+    def fixed(self, name, value):
+        return self.axis(name, [value])
+
+    def _axes(self):
+        keep = []
+        arr = (GridAxis * max(1, len(self.parts)))()
+        for i, (name, values) in enumerate(self.parts):
+            buf = (C.c_double * max(1, len(values)))(*values)
+            keep.append(buf)
+            arr[i] = GridAxis(name.encode(), C.cast(buf, C.POINTER(C.c_double)), len(values))
+        return arr, keep
+
+
+class Record:
+    """The record a scenario fills for a batch of trials (scenario.rs:66-198)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def errors(self, name, errors, total):
+        _check(lib().cf_record_errors(self._h, name.encode(), int(errors), int(total)))
+
+    def value(self, name, v):
+        _check(lib().cf_record_value(self._h, name.encode(), float(v)))
+
+
+STATUS = {0: "completed", 1: "stopped", 2: "skipped", 3: "dev_run"}
+
+
+class ConditionResult:
+    def __init__(self, index, condition, status, trials, wall_s, metrics, confidence):
+        self.index, self.condition, self.status, self.trials, self.wall_s = index, condition, status, trials, wall_s
+        self.metrics, self.confidence = metrics, confidence
+
+    def metric(self, name):
+        return self.metrics.get(name)
+
+
+def _collect(res, conditions, confidence, names):
+    out = []
+    try:
+        for i in range(lib().cf_results_count(res)):
+            st, tr, wall = C.c_int(), C.c_uint64(), C.c_double()
+            _check(lib().cf_results_condition(res, i, C.byref(st), C.byref(tr), C.byref(wall)))
+            metrics = {}
+            for name in names:
+                m = Metric()
+                if lib().cf_results_metric(res, i, name.encode(), confidence, C.byref(m)) == CF_OK:
+                    metrics[name] = m
+            out.append(ConditionResult(i, conditions[i] if i < len(conditions) else {}, STATUS[st.value],
+                                       tr.value, wall.value, metrics, confidence))
+    finally:
+        lib().cf_results_free(res)
+    return out
+
+
+def _conditions(grid):
+    conds = [dict()]
+    for name, values in grid.parts:
+        conds = [dict(c, **{name: v}) for c in conds for v in values]
+    return conds
+
+
+class Runner:
+    """runner.rs:188-473: seed, batch, stop, confidence, fast_dev_run; run(scenario, grid)."""
+
+    def __init__(self):
+        self._seed, self._batch, self._stop, self._conf, self._dev = 0, 32, Stop.max_trials(100), 0.95, False
+
+    def seed(self, s):
+        self._seed = s
+        return self
+
+    def batch(self, n):
+        self._batch = max(1, int(n))
+        return self
+
+    def stop(self, rule):
+        self._stop = rule
+        return self
+
+    def confidence(self, c):
+        self._conf = c
+        return self
+
+    def fast_dev_run(self, on=True):
+        self._dev = bool(on)
+        return self
+
+    def _config(self):
+        return RunnerConfig(self._seed, self._batch, self._conf, int(self._dev))
+
+    def _stop_arr(self):
+        nodes = self._stop.nodes()
+        arr = (StopNode * len(nodes))(*nodes)
+        return arr, len(nodes)
+
+    def run(self, name, batch_fn, grid, metrics=("ber", "fer", "iterations")):
+        """Run a scenario given as batch_fn(index, condition: dict, t0, t1, record)."""
+        errors = []
+        conds = _conditions(grid)
+
+        def cb(user, index, values, n_values, t0, t1, rec):
+            try:
+                batch_fn(index, conds[index], t0, t1, Record(rec))
+                return 0
+            except ComfecError as e:
+                lib().cf_record_fail(rec, e.status, str(e).split(": ", 1)[-1].encode())
+                return 1
+            except Exception as e:  # a Python failure stops the run with its message
+                errors.append(e)
+                lib().cf_record_fail(rec, CF_CONFIG, repr(e).encode())
+                return 1
+
+        fn = BATCH_FN(cb)
+        axes, keep = grid._axes()
+        stop, n_stop = self._stop_arr()
+        cfg = self._config()
+        res = C.c_void_p()
+        status = lib().cf_run(C.byref(cfg), stop, n_stop, axes, len(grid.parts), name.encode(), fn, None,
+                              C.byref(res))
+        if errors:
+            raise errors[0]
+        _check(status)
+        return _collect(res, conds, self._conf, metrics)
+
+    def run_nr(self, code, grid, modulation=BPSK, snr_kind=EBN0_DB, learned_params=None,
+               metrics=("ber", "fer", "iterations")):
+        """Runner::run with the NR link on the GPU as the scenario (cf_nr_campaign)."""
+        link = NrLink(modulation, snr_kind, C.c_void_p(learned_params.data_ptr()) if learned_params is not None
+                      else None)
+        axes, keep = grid._axes()
+        stop, n_stop = self._stop_arr()
+        cfg = self._config()
+        res = C.c_void_p()
+        _check(lib().cf_nr_campaign(code._h, C.byref(link), C.byref(cfg), stop, n_stop, axes, len(grid.parts),
+                                    C.byref(res)))
+        return _collect(res, _conditions(grid), self._conf, metrics)
+
+
+def generate_batch(code, count, modulation, snr_db, snr_kind=EBN0_DB, seed=1, point=0, trial0=0, stream=None):
+    """The device trial generator: (LLRs (count, E) float32, packed info bits (count, ceil(K'/32)) int32)."""
+    import torch
+    llr = torch.empty((count, code.dims.e), dtype=torch.float32, device="cuda")
+    info = torch.empty((count, (code.dims.k_info + 31) // 32), dtype=torch.int32, device="cuda")
+    _check(lib().cf_generate_batch(code._h, modulation, snr_kind, snr_db, seed, point, trial0, count,
+                                   C.c_void_p(llr.data_ptr()), C.c_void_p(info.data_ptr()), _stream_ptr(stream)))
+    return llr, info
+
+
+def unpack_info(info, k_info):
+    """Packed information bits (LSB first) to a (count, K') uint8 array."""
+    w = np.ascontiguousarray(info.cpu().numpy() if hasattr(info, "cpu") else info).view(np.uint32)
+    bits = (w[:, :, None] >> np.arange(32, dtype=np.uint32)) & 1
+    return bits.reshape(w.shape[0], -1)[:, :k_info].astype(np.uint8)

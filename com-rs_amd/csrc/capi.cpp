@@ -11,44 +11,18 @@
 #include <utility>
 #include <vector>
 
+#include "capi_internal.hpp"
 #include "kernels.hpp"
 #include "nr_code.hpp"
 
 using namespace comfec;
 
-struct cf_code {
-    NrCode code;
-    mutable std::mutex mu;  // guards the decoder settings
-    explicit cf_code(NrCode&& c) : code(std::move(c)) {}
-    DecoderSettings settings() const {
-        std::lock_guard<std::mutex> l(mu);
-        return code.dec;
-    }
-};
-
-namespace {
+namespace comfec {
+namespace detail {
 
 thread_local std::string g_error;
 
 cf_status status_of(Kind k) { return static_cast<cf_status>(static_cast<int>(k)); }
-
-template <typename F>
-cf_status guarded(F&& f) {
-    try {
-        f();
-        g_error.clear();
-        return CF_OK;
-    } catch (const Error& e) {
-        g_error = e.what();
-        return status_of(e.kind);
-    } catch (const std::bad_alloc&) {
-        g_error = "device: out of host memory";
-        return CF_DEVICE;
-    } catch (const std::exception& e) {
-        g_error = std::string("device: ") + e.what();
-        return CF_DEVICE;
-    }
-}
 
 void check_hip(hipError_t err, const char* what) {
     if (err != hipSuccess) fail(Kind::Device, std::string(what) + ": " + hipGetErrorString(err));
@@ -184,7 +158,10 @@ void decode_host_f64(const cf_code* h, const double* llr, bool full, const doubl
     if (bits_host) check_hip(hipMemcpy(bits_host, d_bits, c.k_info, hipMemcpyDeviceToHost), "copy out");
 }
 
-}  // namespace
+}  // namespace detail
+}  // namespace comfec
+
+using namespace comfec::detail;
 
 extern "C" {
 

@@ -1,0 +1,54 @@
+// Internals shared by the C ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+
+#include "../../include/comfec_gpu.h"
+#include "kernels.hpp"
+#include "nr_code.hpp"
+
+struct cf_code {
+    comfec::NrCode code;
+    mutable std::mutex mu;  // guards the decoder settings
+    explicit cf_code(comfec::NrCode&& c) : code(std::move(c)) {}
+    comfec::DecoderSettings settings() const {
+        std::lock_guard<std::mutex> l(mu);
+        return code.dec;
+    }
+};
+
+
+namespace comfec {
+namespace detail {
+
+extern thread_local std::string g_error;
+cf_status status_of(Kind k);
+void check_hip(hipError_t err, const char* what);
+int current_device();
+bool fast_layered(const cf_code* h);
+void run_layered(const cf_code* h, int device, const float* llr, int64_t count, uint8_t* bits, float* post,
+                 uint32_t* iterations, const float* params, hipStream_t stream);
+
+// Runs f, mapping a thrown Error to its status and the thread's last message.
+template <typename F>
+cf_status guarded(F&& f) {
+    try {
+        f();
+        g_error.clear();
+        return CF_OK;
+    } catch (const Error& e) {
+        g_error = e.what();
+        return status_of(e.kind);
+    } catch (const std::bad_alloc&) {
+        g_error = "device: out of host memory";
+        return CF_DEVICE;
+    } catch (const std::exception& e) {
+        g_error = std::string("device: ") + e.what();
+        return CF_DEVICE;
+    }
+}
+
+}  // namespace detail
+}  // namespace comfec

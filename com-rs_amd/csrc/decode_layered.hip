@@ -50,6 +50,33 @@ constexpr int row_begin(int i) {
     return s;
 }
 
+// Whether layer I must wait for the layers before it: it shares a column with a layer
+// processed since the last barrier (reads after writes; and writes after reads, which are
+// the same shared columns). Layers on disjoint columns commute, so they run without one.
+template <int BG>
+constexpr bool shares_column(int i, int l) {
+    for (int e = 0; e < Graph<BG>::edges; ++e)
+        if (Graph<BG>::t[e].i == i)
+            for (int f = 0; f < Graph<BG>::edges; ++f)
+                if (Graph<BG>::t[f].i == l && Graph<BG>::t[f].j == Graph<BG>::t[e].j) return true;
+    return false;
+}
+
+template <int BG>
+constexpr bool barrier_before(int layer) {
+    if (layer == 0) return false;
+    int first = 0;  // first layer after the last barrier
+    for (int l = 1; l <= layer; ++l) {
+        bool conflict = false;
+        for (int p = first; p < l; ++p) conflict = conflict || shares_column<BG>(l, p);
+        if (conflict) {
+            if (l == layer) return true;
+            first = l;
+        }
+    }
+    return false;
+}
+
 // The messages of one lane's rows: output magnitudes g(m1), g(m2) and, packed, the
 // output sign bits (bit q of edge q) and the index of the first minimum (bits 24..28).
 template <int ROWS>
@@ -117,8 +144,8 @@ __device__ __forceinline__ void sweep(float* post, int r, int z, bool active, co
             a = ab[2 * I];
             b = ab[2 * I + 1];
         }
+        if constexpr (barrier_before<BG>(I)) __syncthreads();
         if (active) layer<BG, I>(post, r, z, shift, msg, a, b);
-        __syncthreads();
         sweep<BG, I + 1>(post, r, z, active, shift, msg, ab, a, b);
     }
 }
@@ -206,6 +233,7 @@ __global__ void __launch_bounds__(384) decode_layered_kernel(LayeredArgs a) {
     for (int it = 1; it <= a.max_iterations && done < 0; ++it) {
         const float* ab = a.params ? a.params + size_t(it - 1) * 2 * Graph<BG>::rows : nullptr;
         sweep<BG, 0>(post, r, z, active, a.shift, msg, ab, a.a, a.b);
+        __syncthreads();
         if (!syndrome_fails<BG>(post, hd, a)) done = it;
     }
     if (done < 0) done = a.max_iterations;

@@ -61,6 +61,44 @@ struct LayeredArgs {
 bool layered_supported(int bg, int z);
 hipError_t launch_layered(const LayeredArgs& a, int bg, hipStream_t stream);
 
+// The device trial generator: information bits, encoder, rate matching, channel, LLRs.
+struct TrialGenArgs {
+    int32_t z, n, k, k_info, e, rows_b, core_words;
+    const uint16_t* row_start;
+    const uint16_t* col;
+    const uint16_t* shift;
+    const uint32_t* map;       // e: codeword bit of every transmitted bit
+    const uint64_t* core_inv;  // 4Z rows of core_words
+    int bits_per_symbol;       // 1: BPSK on a real channel; 2, 4, 6: Gray QAM
+    float noise_var;           // BPSK: per real dimension; QAM: N0 per complex symbol
+    float noise_sd;            // BPSK: sqrt(noise_var); QAM: sqrt(N0 / 2) per component
+    float log_prior;           // -ln M (the uniform prior of the max-log metric)
+    float pts_re[64], pts_im[64];
+    uint32_t k0, k1;           // the 64-bit seed
+    uint32_t point;
+    uint64_t trial0;
+    int64_t count;
+    float* llr;                // count x e
+    uint32_t* info;            // count x ceil(K'/32), LSB first
+};
+
+// The error counts of a decoded batch.
+struct CountArgs {
+    int32_t k_info;
+    const uint8_t* bits;       // count x K'
+    const uint32_t* info;      // count x ceil(K'/32)
+    const uint32_t* iterations;
+    int64_t count;
+    uint64_t* counters;        // [bit errors, frame errors, sum of iterations, sum of their squares]
+    uint32_t* per_trial_errors;
+};
+
+size_t generate_lds_bytes(int z, int n);
+hipError_t launch_generate(const TrialGenArgs& a, hipStream_t stream);
+hipError_t launch_count(const CountArgs& a, hipStream_t stream);
+// a fill kernel (not a memset node, so that graph replays clear the counters too)
+hipError_t launch_zero(uint64_t* p, int n, hipStream_t stream);
+
 // A grow-only device buffer, one per (device, stream) in the C ABI layer.
 struct Workspace {
     void* ptr = nullptr;

@@ -252,7 +252,9 @@ const NrCode::DeviceTables& NrCode::device_tables(int device) const {
     // one buffer: u16 tables, then u32 tables (4-byte aligned)
     size_t n16 = row_start.size() + col.size() + shift.size() + col_start.size() + col_edge.size();
     n16 = (n16 + 1) & ~size_t(1);
-    size_t bytes = n16 * 2 + (src_start.size() + src.size()) * 4;
+    size_t n32 = src_start.size() + src.size() + map.size();
+    n32 = (n32 + 1) & ~size_t(1);
+    size_t bytes = n16 * 2 + n32 * 4 + core_inv.size() * 8;
     std::vector<uint8_t> host(bytes, 0);
     uint16_t* h16 = reinterpret_cast<uint16_t*>(host.data());
     size_t o = 0;
@@ -267,6 +269,9 @@ const NrCode::DeviceTables& NrCode::device_tables(int device) const {
     uint32_t* h32 = reinterpret_cast<uint32_t*>(host.data() + n16 * 2);
     std::memcpy(h32, src_start.data(), src_start.size() * 4);
     std::memcpy(h32 + src_start.size(), src.data(), src.size() * 4);
+    std::memcpy(h32 + src_start.size() + src.size(), map.data(), map.size() * 4);
+    uint64_t* h64 = reinterpret_cast<uint64_t*>(host.data() + n16 * 2 + n32 * 4);
+    std::memcpy(h64, core_inv.data(), core_inv.size() * 8);
     int prev = 0;
     if (hipGetDevice(&prev) != hipSuccess) fail(Kind::Device, "hipGetDevice failed");
     if (prev != device && hipSetDevice(device) != hipSuccess) fail(Kind::Device, "hipSetDevice failed");
@@ -288,6 +293,8 @@ const NrCode::DeviceTables& NrCode::device_tables(int device) const {
     t.col_edge = d16 + o_ce;
     t.src_start = d32;
     t.src = d32 + src_start.size();
+    t.map = d32 + src_start.size() + src.size();
+    t.core_inv = reinterpret_cast<const uint64_t*>(static_cast<const uint8_t*>(base) + n16 * 2 + n32 * 4);
     return t;
 }
 

@@ -79,6 +79,8 @@ struct NrCode {
         const uint16_t* col_edge = nullptr;
         const uint32_t* src_start = nullptr;
         const uint32_t* src = nullptr;
+        const uint32_t* map = nullptr;
+        const uint64_t* core_inv = nullptr;
     };
     const DeviceTables& device_tables(int device) const;
     ~NrCode();

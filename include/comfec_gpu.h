@@ -114,6 +114,107 @@ cf_status cf_app_into(const cf_code* code, const double* llr_host, size_t e,
 cf_status cf_decode_batch(const cf_code* code, int msg_type, const void* d_llr, size_t count,
                           uint8_t* d_bits, void* d_post, uint32_t* d_iterations, void* stream);
 
+
+/* ---- the device trial generator ------------------------------------------------------ */
+
+/* Modulations of the generator: BPSK on a real channel, or Gray square QAM (QPSK, 16-QAM,
+   64-QAM) normalized to unit energy (crates/com-mapping/src/constellation.rs:168-205). */
+enum { CF_BPSK = 1, CF_QPSK = 2, CF_QAM16 = 4, CF_QAM64 = 6 };
+/* How the noise is given: Eb/N0 in dB at the code rate K'/E, or Es/N0 in dB. */
+enum { CF_EBN0_DB = 0, CF_ESN0_DB = 1 };
+
+/* No reference counterpart as one call: the body of Scenario::trial of the coded links
+   (validation/tests/aff3ct.rs:151-158 with source.rs:30-32, encode_block, bpsk_awgn_llr
+   validation/src/lib.rs:67-71, awgn.rs:94-99 and the max-log LLRs demapper.rs:270-297) for
+   trials trial0 .. trial0+count-1 of grid point `point`, keyed by Philox-4x32-10 counters
+   under `seed`. Writes count x E float32 LLRs and count x ceil(K'/32) packed information
+   bits (LSB first) to device memory. */
+cf_status cf_generate_batch(const cf_code* code, int modulation, int snr_kind, double snr_db,
+                            uint64_t seed, uint32_t point, uint64_t trial0, size_t count,
+                            float* d_llr, uint32_t* d_info, void* stream);
+
+/* ---- campaigns: crates/com-sim ------------------------------------------------------- */
+
+/* Record (crates/com-sim/src/scenario.rs:66-198): the metrics of a batch of trials. */
+typedef struct cf_record cf_record;
+/* Record::errors (scenario.rs:90-97) and Record::value (scenario.rs:104-115) */
+cf_status cf_record_errors(cf_record* rec, const char* name, uint64_t errors, uint64_t total);
+cf_status cf_record_value(cf_record* rec, const char* name, double value);
+
+/* Stop (crates/com-sim/src/stopping.rs:64-141) as its nodes in prefix order: an ANY (or)
+   or ALL (and) node is followed by its n_children subtrees. */
+enum { CF_STOP_MAX_TRIALS = 0, CF_STOP_MIN_TRIALS = 1, CF_STOP_MIN_ERRORS = 2,
+       CF_STOP_PRECISION = 3, CF_STOP_MAX_TIME = 4, CF_STOP_ANY = 5, CF_STOP_ALL = 6 };
+typedef struct {
+    int kind;
+    uint32_t n_children;  /* ANY / ALL */
+    uint64_t count;       /* MAX_TRIALS, MIN_TRIALS: trials; MIN_ERRORS: errors */
+    double value;         /* PRECISION: relative half-width; MAX_TIME: seconds */
+    double confidence;    /* PRECISION */
+    const char* metric;   /* MIN_ERRORS, PRECISION */
+} cf_stop_node;
+
+/* Grid::axis / Grid::fixed (crates/com-sim/src/grid.rs:231-290): axes in order, the last
+   fastest; a fixed value is an axis of one value. */
+typedef struct {
+    const char* name;
+    const double* values;
+    size_t n_values;
+} cf_grid_axis;
+
+/* The Runner builder (crates/com-sim/src/runner.rs:224-276). */
+typedef struct {
+    uint64_t seed;
+    uint64_t batch;
+    double confidence;
+    int fast_dev_run;
+} cf_runner_config;
+
+/* Scenario::setup + Scenario::trial for the trials [t0, t1) of condition `index` whose
+   values are cond_values (in axis order): record their metrics in `rec` and return 0, or
+   return nonzero after cf_record_fail to stop the run with that error. */
+typedef int (*cf_batch_fn)(void* user, size_t index, const double* cond_values, size_t n_values,
+                           uint64_t t0, uint64_t t1, cf_record* rec);
+cf_status cf_record_fail(cf_record* rec, int kind, const char* message);
+
+typedef struct cf_results cf_results;
+
+/* Runner::run (runner.rs:309-473) on a scenario given as a callback. */
+cf_status cf_run(const cf_runner_config* config, const cf_stop_node* stop, size_t n_stop,
+                 const cf_grid_axis* axes, size_t n_axes, const char* scenario_name,
+                 cf_batch_fn scenario, void* user, cf_results** out);
+
+/* The NR link of a campaign on the GPU: per trial, K' random bits, the NR encoder, the
+   modulation over AWGN at the condition's SNR (grid key "ebn0_db" or "esn0_db"), the
+   device decoder with the code's settings (or the learned offset/scale min-sum when
+   d_learned_params is set: (a, b) per iteration and base row, float32), and the records
+   "ber" (bit errors over K' bits), "fer" (frame errors) and "iterations" (mean). */
+typedef struct {
+    int modulation;               /* CF_BPSK, CF_QPSK, CF_QAM16, CF_QAM64 */
+    int snr_kind;                 /* CF_EBN0_DB or CF_ESN0_DB */
+    const float* d_learned_params;
+} cf_nr_link;
+
+/* Runner::run with the NR link as the scenario: batches of `batch` trials on the GPU. */
+cf_status cf_nr_campaign(const cf_code* code, const cf_nr_link* link, const cf_runner_config* config,
+                         const cf_stop_node* stop, size_t n_stop, const cf_grid_axis* axes,
+                         size_t n_axes, cf_results** out);
+
+/* ConditionResult (runner.rs:71-99) and ConditionResult::metric (runner.rs:101-104). */
+typedef struct {
+    double value, lo, hi;   /* estimate and its interval at the requested confidence */
+    uint64_t count, total;  /* errors and trials (rates), or values twice (means) */
+    int is_rate;
+    uint64_t dropped;
+} cf_metric;
+size_t cf_results_count(const cf_results* res);
+/* status: 0 completed, 1 stopped, 2 skipped, 3 dev run (runner.rs:58-68) */
+cf_status cf_results_condition(const cf_results* res, size_t i, int* status, uint64_t* trials,
+                               double* wall_s);
+cf_status cf_results_metric(const cf_results* res, size_t i, const char* name, double confidence,
+                            cf_metric* out);
+void cf_results_free(cf_results* res);
+
 #ifdef __cplusplus
 }
 #endif

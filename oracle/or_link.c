@@ -296,3 +296,36 @@ void or_wilson(uint64_t errors, uint64_t total, double confidence, double* lo, d
     *lo = lower;
     *hi = fmin(centre + half, 1.0);
 }
+
+/* Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): the counter-based generator of the
+   device trial generator (no reference counterpart), restated for its known-answer tests
+   and to regenerate the device's information bits. */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1, n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* The K' information bits of trial `trial` at grid point `point` of the device generator:
+   block b of 128 bits is Philox(counter = (b, trial low, trial high, point*16 + 0), seed). */
+void or_philox_info_bits(uint64_t seed, uint32_t point, uint64_t trial, int32_t k_info, uint8_t* bits) {
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    int32_t words = (k_info + 31) / 32;
+    for (int32_t blk = 0; blk * 4 < words; ++blk) {
+        uint32_t ctr[4] = {(uint32_t)blk, (uint32_t)trial, (uint32_t)(trial >> 32), point << 4};
+        uint32_t out[4];
+        or_philox4x32_10(ctr, key, out);
+        for (int q = 0; q < 4; ++q)
+            for (int b = 0; b < 32; ++b) {
+                int32_t p = (blk * 4 + q) * 32 + b;
+                if (p < k_info) bits[p] = (uint8_t)((out[q] >> b) & 1u);
+            }
+    }
+}

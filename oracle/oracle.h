@@ -93,6 +93,10 @@ void or_qam_awgn_llr(int m, const uint8_t* bits, int64_t nbits, double noise_var
 void or_qam_llr(int m, const double* yre, const double* yim, int64_t nsym, double noise_var,
                 double* llr);
 
+/* Philox-4x32-10 and the information bits of the device trial generator */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void or_philox_info_bits(uint64_t seed, uint32_t point, uint64_t trial, int32_t k_info, uint8_t* bits);
+
 /* ---- statistics: crates/com-math/src/special.rs, crates/com-sim/src/stats.rs ---------- */
 double or_erfc(double x);                       /* special.rs:180-255 (Cody) */
 double or_erfcinv(double y);                    /* special.rs:265-273 */

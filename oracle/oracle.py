@@ -73,6 +73,8 @@ def _load():
         "or_qam_points": (None, [C.c_int, _f64p, _f64p]),
         "or_qam_llr": (None, [C.c_int, _f64p, _f64p, C.c_int64, C.c_double, _f64p]),
         "or_qam_awgn_llr": (None, [C.c_int, _u8p, C.c_int64, C.c_double, C.POINTER(OrRng), _f64p]),
+        "or_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+        "or_philox_info_bits": (None, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_int32, _u8p]),
         "or_erfc": (C.c_double, [C.c_double]),
         "or_erfcinv": (C.c_double, [C.c_double]),
         "or_qfuncinv": (C.c_double, [C.c_double]),
@@ -249,6 +251,20 @@ def wilson(errors, total, confidence=0.95):
     lo, hi = C.c_double(), C.c_double()
     lib.or_wilson(errors, total, confidence, C.byref(lo), C.byref(hi))
     return lo.value, hi.value
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib.or_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def philox_info_bits(seed, point, trial, k_info):
+    bits = np.zeros(k_info, np.uint8)
+    lib.or_philox_info_bits(seed, point, trial, k_info, bits)
+    return bits
 
 
 def qam_points(m):

@@ -1,0 +1,121 @@
+"""The device trial generator and the GPU campaign, through the C ABI.
+
+- information bits: bit-exact with the oracle's Philox-4x32-10 (pinned by its known answers);
+- encoder, rate matching, interleaver and modulation: at a very high SNR the LLR signs are
+  the oracle's encoded bits exactly;
+- noise: LLR moments against the closed form of BPSK over AWGN;
+- campaigns: frame error rates inside the 99 % Wilson interval of the oracle's run of the
+  reference trial chain (xoshiro streams), identical records whatever the batch size.
+"""
+import numpy as np
+import pytest
+
+import comfec as F
+import oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def test_info_bits_are_the_philox_stream():
+    print("\n[generator] info bits vs Philox", flush=True)
+    code = F.NrLdpc.lifted(2, 52)
+    llr, info = F.generate_batch(code, 32, F.BPSK, 2.0, seed=7, point=3, trial0=100)
+    bits = F.unpack_info(info, code.k)
+    for i in range(32):
+        assert np.array_equal(bits[i], O.philox_info_bits(7, 3, 100 + i, code.k))
+    # a trial's draws do not depend on the batch it is generated in
+    llr2, info2 = F.generate_batch(code, 8, F.BPSK, 2.0, seed=7, point=3, trial0=110)
+    assert torch.equal(info2, info[10:18]) and torch.equal(llr2, llr[10:18])
+
+
+@pytest.mark.parametrize("case", [("lifted BG2 Z=52 BPSK", None, F.BPSK), ("NR BG1 Z=384 16-QAM qm=4", (8448, 25344, 1, 0, 4),
+                                   F.QAM16), ("NR BG1 Z=384 QPSK qm=2", (8448, 25344, 1, 0, 2), F.QPSK),
+                                  ("NR K'=500 E=1200 rv2 16-QAM", (500, 1200, 0, 2, 4), F.QAM16),
+                                  ("NR K'=1000 E=1500 rv3 64-QAM", (1000, 1500, 0, 3, 6), F.QAM64)],
+                         ids=lambda c: c[0])
+def test_noiseless_llr_signs_are_the_encoded_bits(case):
+    name, args, mod = case
+    print(f"\n[generator] {name}", flush=True)
+    if args is None:
+        code, ref = F.NrLdpc.lifted(2, 52), O.Code.lifted(2, 52)
+    else:
+        code, ref = F.NrLdpc(*args), O.Code.nr(args[0], args[1], args[2], args[3], args[4])
+    llr, info = F.generate_batch(code, 6, mod, 60.0, snr_kind=F.ESN0_DB, seed=11, point=0, trial0=0)
+    llr = llr.cpu().numpy()
+    bits = F.unpack_info(info, code.k)
+    for i in range(6):
+        if args is None:
+            tx = ref.encode(bits[i])[2 * ref.z:]
+        else:
+            tx = ref.nr_encode(bits[i])
+        assert np.array_equal((llr[i] < 0).astype(np.uint8), tx)
+        assert np.all(np.abs(llr[i]) > 1.0)
+
+
+def test_bpsk_llr_moments():
+    """LLR = 2y/s2 with y = x + s n: times (1 - 2c) it is N(2/s2, 4/s2)."""
+    print("\n[generator] BPSK moments", flush=True)
+    code, ref = F.NrLdpc.lifted(2, 52), O.Code.lifted(2, 52)
+    llr, info = F.generate_batch(code, 400, F.BPSK, 2.0, seed=3)
+    llr = llr.cpu().numpy().astype(np.float64)
+    bits = F.unpack_info(info, code.k)
+    tx = np.array([ref.encode(b)[104:] for b in bits])
+    v = llr * (1.0 - 2.0 * tx)
+    s2 = O.lib.or_bpsk_noise_variance(2.0, 520 / 2600)
+    assert abs(v.mean() - 2 / s2) < 0.01 * 2 / s2
+    assert abs(v.var() - 4 / s2) < 0.02 * 4 / s2
+
+
+def oracle_fer(code, ebn0, trials, iters=20):
+    """The reference trial chain on the CPU: xoshiro streams, BPSK, f64 layered NMS 0.75."""
+    fulls, cws = [], []
+    for t in range(trials):
+        cw, llr = code.trial_bpsk_lifted(ebn0, 1, "ref", 0, t)
+        fulls.append(llr)
+        cws.append(cw)
+    post, its = code.decode_many(np.array(fulls), rule=O.NMS, alpha=0.75, max_iterations=iters, threads=8)
+    errs = ((post[:, :code.k] < 0).astype(np.uint8) != np.array(cws)[:, :code.k]).sum(1)
+    return int((errs > 0).sum()), int(errs.sum()), float(its.mean())
+
+
+def test_campaign_fer_matches_the_reference_chain():
+    print("\n[campaign] BG2 Z=52 BPSK 1.25 dB vs the oracle", flush=True)
+    code = F.NrLdpc.lifted(2, 52).set_decoder("normalized_min_sum", 0.75, True, 20)
+    res = F.Runner().seed(7).batch(4096).stop(F.Stop.max_trials(16384)).run_nr(
+        code, F.Grid().axis("ebn0_db", [1.25]), F.BPSK, F.EBN0_DB)
+    fer = res[0].metric("fer")
+    ber = res[0].metric("ber")
+    assert res[0].trials == 16384 and fer.total == 16384 and ber.total == 16384 * 520
+    ref = O.Code.lifted(2, 52)
+    fe, be, it = oracle_fer(ref, 1.25, 3000)
+    lo, hi = O.wilson(fe, 3000, 0.99)
+    assert lo <= fer.value <= hi, (fer.value, fe / 3000)
+    blo, bhi = O.wilson(be, 3000 * 520, 0.99)
+    # bit errors cluster in frames: widen the BER check by the frame spread
+    assert blo / 1.5 <= ber.value <= bhi * 1.5
+    assert abs(res[0].metric("iterations").value - it) < 0.5
+
+
+def test_campaign_records_do_not_depend_on_the_batch():
+    print("\n[campaign] batch independence", flush=True)
+    code = F.NrLdpc.lifted(2, 52).set_decoder("normalized_min_sum", 0.75, True, 20)
+    grid = F.Grid().axis("ebn0_db", [1.0, 1.5])
+    out = []
+    for batch in (3000, 1000, 777):
+        r = F.Runner().seed(5).batch(batch).stop(F.Stop.max_trials(3000)).run_nr(code, grid)
+        out.append([(c.trials, c.metric("fer").count, c.metric("ber").count, c.metric("iterations").value)
+                    for c in r])
+    assert out[0] == out[1] == out[2]
+    assert out[0][0][1] > out[0][1][1]  # more frame errors at the lower SNR
+
+
+def test_campaign_stops_on_frame_errors():
+    print("\n[campaign] min_errors stop", flush=True)
+    code = F.NrLdpc(8448, 25344, 1, 0, 2).set_decoder("normalized_min_sum", 0.75, True, 10)
+    r = F.Runner().seed(7).batch(512).stop(F.Stop.min_errors("fer", 50) | F.Stop.max_trials(20000)).run_nr(
+        code, F.Grid().axis("ebn0_db", [0.5, 1.0]), F.QPSK, F.EBN0_DB)
+    for c in r:
+        m = c.metric("fer")
+        assert c.status == "completed" and c.trials % 512 == 0 or c.trials == 20000
+        assert m.count >= 50 or c.trials == 20000
