@@ -152,6 +152,7 @@ class CallbackScenario : public sim::BatchScenario {
     void run_batch(const sim::Condition& cond, size_t index, uint64_t t0, uint64_t t1, sim::Record& rec) override {
         std::vector<double> values;
         for (const auto& e : cond.entries) values.push_back(e.second);
+        // the callback's record of the batch, merged as one
         cf_record r;
         const int rc = fn_(user_, index, values.data(), values.size(), t0, t1, &r);
         if (rc != 0 || r.failed) {
@@ -198,6 +199,7 @@ class NrLinkScenario : public sim::BatchScenario {
     void run_batch(const sim::Condition&, size_t index, uint64_t t0, uint64_t t1, sim::Record& rec) override {
         const NrCode& c = h_->code;
         uint64_t counts[4] = {0, 0, 0, 0};
+        std::vector<uint32_t> its(size_t(std::min<uint64_t>(cap_, t1 - t0)));
         for (uint64_t s = t0; s < t1; s += cap_) {
             const uint64_t n = std::min<uint64_t>(cap_, t1 - s);
             TrialGenArgs g = generator_args(c, device_, link_.modulation, link_.snr_kind, snr_, seed_, uint32_t(index));
@@ -226,24 +228,21 @@ class NrLinkScenario : public sim::BatchScenario {
             check_hip(launch_count(ca, stream_), "count launch");
             uint64_t part[4];
             check_hip(hipMemcpyAsync(part, counters_.p, sizeof part, hipMemcpyDeviceToHost, stream_), "copy");
+            check_hip(hipMemcpyAsync(its.data(), its_.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
+                      "copy");
             check_hip(hipStreamSynchronize(stream_), "batch");
             for (int q = 0; q < 4; ++q) counts[q] += part[q];
+            // the iteration count of every trial, merged in trial order as one-value means
+            for (uint64_t i = 0; i < n; ++i) {
+                sim::Record one;
+                one.value("iterations", double(its[i]));
+                rec.merge(one);
+            }
         }
         const uint64_t trials = t1 - t0;
+        // error counts are integer sums: the batch total is the sum of the per-trial records
         rec.errors("ber", counts[0], trials * c.k_info);
         rec.errors("fer", counts[1], trials);
-        // the mean of the iteration counts, from their integer sums (Welford's state)
-        sim::Mean m;
-        m.n = trials;
-        m.mean = double(counts[2]) / double(trials);
-        m.m2 = double(counts[3]) - double(counts[2]) * double(counts[2]) / double(trials);
-        if (m.m2 < 0.0) m.m2 = 0.0;
-        sim::Record its;
-        sim::Estimate e;
-        e.is_rate = false;
-        e.mean = m;
-        its.metrics.emplace("iterations", e);
-        rec.merge(its);
     }
 
   private:

@@ -455,14 +455,12 @@ std::vector<ConditionResult> Runner::run(BatchScenario& scenario, const Grid& gr
         uint64_t trials = 0;
         for (;;) {
             const uint64_t end = bounded ? std::max(std::min(trials + batch_, limit), trials + 1) : trials + batch_;
-            Record rec;
             try {
-                scenario.run_batch(cond, index, trials, end, rec);
+                scenario.run_batch(cond, index, trials, end, res.record);
             } catch (const Error& e) {
                 throw with_context(e, "trials " + std::to_string(trials) + ".." + std::to_string(end) + " of (" +
                                           cond.label() + ")");
             }
-            res.record.merge(rec);
             if (trials == 0) {
                 try {
                     stop_.check_metrics(res.record);

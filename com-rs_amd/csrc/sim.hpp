@@ -136,7 +136,9 @@ class BatchScenario {
     virtual ~BatchScenario() = default;
     virtual std::string name() const = 0;
     virtual void setup(const Condition& cond, size_t index) = 0;
-    // trials [t0, t1) of condition `index`, merged into `rec` (in trial order for means)
+    // trials [t0, t1) of condition `index`, merged into the condition's record `rec` in
+    // trial order (runner.rs:448-450), so that means come out as the per-trial merges of the
+    // reference whatever the batch size
     virtual void run_batch(const Condition& cond, size_t index, uint64_t t0, uint64_t t1, Record& rec) = 0;
 };
 
