@@ -468,3 +468,79 @@ def unpack_info(info, k_info):
     w = np.ascontiguousarray(info.cpu().numpy() if hasattr(info, "cpu") else info).view(np.uint32)
     bits = (w[:, :, None] >> np.arange(32, dtype=np.uint32)) & 1
     return bits.reshape(w.shape[0], -1)[:, :k_info].astype(np.uint8)
+
+
+# ---- the learned offset/scale min-sum ---------------------------------------------------------
+
+SIGNATURES += [
+    ("cf_generate_training_batch", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint64,
+                                             C.c_uint32, C.c_uint64, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("cf_learned_loss_grad", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                       C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("cf_adam_step", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint32, C.c_double,
+                               C.c_double, C.c_double, C.c_double, C.c_void_p]),
+    ("cf_learned_train", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_double, C.c_int,
+                                   C.c_int, C.c_double, C.c_double, C.c_uint64, C.c_void_p, C.c_void_p]),
+    ("cf_learned_decode_batch", C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_int, C.c_void_p, C.c_size_t,
+                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+]
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def initial_params(code, iterations, alpha=0.75, beta=0.0):
+    """(a, b) = (alpha, beta) for every iteration and base row: the plain min-sum."""
+    import torch
+    rows = 46 if code.dims.base_graph == 1 else 42
+    p = torch.empty((iterations, rows, 2), dtype=torch.float32, device="cuda")
+    p[..., 0] = alpha
+    p[..., 1] = beta
+    return p
+
+
+def generate_training_batch(code, count, modulation, snr_lo, snr_hi, snr_kind=EBN0_DB, seed=1, point=0, trial0=0,
+                            stream=None):
+    """Training LLRs with per-trial SNR uniform in [lo, hi], and the packed codeword bits."""
+    import torch
+    llr = torch.empty((count, code.dims.e), dtype=torch.float32, device="cuda")
+    cw = torch.empty((count, (code.dims.n + 31) // 32), dtype=torch.int32, device="cuda")
+    _check(lib().cf_generate_training_batch(code._h, modulation, snr_kind, snr_lo, snr_hi, seed, point, trial0, count,
+                                            _ptr(llr), _ptr(cw), _stream_ptr(stream)))
+    return llr, cw
+
+
+def learned_loss_grad(code, params, llr, cw, stream=None):
+    import torch
+    grad = torch.empty_like(params)
+    loss = torch.empty(1, dtype=torch.float32, device="cuda")
+    _check(lib().cf_learned_loss_grad(code._h, params.shape[0], _ptr(params), _ptr(llr), _ptr(cw), llr.shape[0],
+                                      _ptr(grad), _ptr(loss), _stream_ptr(stream)))
+    return loss, grad
+
+
+def adam_step(params, grad, m, v, step, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, stream=None):
+    _check(lib().cf_adam_step(_ptr(params), _ptr(grad), _ptr(m), _ptr(v), params.numel(), step, lr, beta1, beta2,
+                              eps, _stream_ptr(stream)))
+
+
+def learned_train(code, params, steps, batch, lr=1e-3, modulation=BPSK, snr_kind=EBN0_DB, snr_lo=0.0, snr_hi=3.0,
+                  seed=1, stream=None):
+    """Train in place; returns the loss of every step (numpy)."""
+    hist = np.zeros(steps, np.float32)
+    _check(lib().cf_learned_train(code._h, params.shape[0], _ptr(params), steps, batch, lr, modulation, snr_kind,
+                                  snr_lo, snr_hi, seed, hist.ctypes.data_as(C.c_void_p), _stream_ptr(stream)))
+    return hist
+
+
+def learned_decode_batch(code, params, llr, fallback=True, stream=None):
+    """(bits (count, K'), iterations (count,), ok (count,)) with the learned parameters."""
+    import torch
+    count = llr.shape[0]
+    bits = torch.empty((count, code.dims.k_info), dtype=torch.uint8, device="cuda")
+    its = torch.empty(count, dtype=torch.int32, device="cuda")
+    ok = torch.empty(count, dtype=torch.uint8, device="cuda")
+    _check(lib().cf_learned_decode_batch(code._h, params.shape[0], _ptr(params), int(fallback), _ptr(llr), count,
+                                         _ptr(bits), _ptr(its), _ptr(ok), _stream_ptr(stream)))
+    return bits, its, ok

@@ -59,8 +59,13 @@ void qam_points(int bps, float* re, float* im) {
     }
 }
 
-TrialGenArgs generator_args(const NrCode& c, int device, int modulation, int snr_kind, double snr_db, uint64_t seed,
-                            uint32_t point) {
+}  // namespace
+
+namespace comfec {
+namespace detail {
+
+TrialGenArgs make_generator_args(const NrCode& c, int device, int modulation, int snr_kind, double snr_db,
+                                 uint64_t seed, uint32_t point) {
     if (modulation != CF_BPSK && modulation != CF_QPSK && modulation != CF_QAM16 && modulation != CF_QAM64)
         fail(Kind::Config, "the modulation must be BPSK, QPSK, 16-QAM or 64-QAM, got " + std::to_string(modulation));
     if (snr_kind != CF_EBN0_DB && snr_kind != CF_ESN0_DB)
@@ -103,6 +108,11 @@ TrialGenArgs generator_args(const NrCode& c, int device, int modulation, int snr
     a.point = point;
     return a;
 }
+
+}  // namespace detail
+}  // namespace comfec
+
+namespace {
 
 sim::Stop stop_from_nodes(const cf_stop_node* nodes, size_t n, size_t& pos) {
     if (pos >= n) fail(Kind::Config, "the stopping rule ends early");
@@ -202,7 +212,7 @@ class NrLinkScenario : public sim::BatchScenario {
         std::vector<uint32_t> its(size_t(std::min<uint64_t>(cap_, t1 - t0)));
         for (uint64_t s = t0; s < t1; s += cap_) {
             const uint64_t n = std::min<uint64_t>(cap_, t1 - s);
-            TrialGenArgs g = generator_args(c, device_, link_.modulation, link_.snr_kind, snr_, seed_, uint32_t(index));
+            TrialGenArgs g = make_generator_args(c, device_, link_.modulation, link_.snr_kind, snr_, seed_, uint32_t(index));
             g.trial0 = s;
             g.count = int64_t(n);
             g.llr = llr_.p;
@@ -270,7 +280,7 @@ cf_status cf_generate_batch(const cf_code* h, int modulation, int snr_kind, doub
     return guarded([&] {
         if (!h) fail(Kind::Config, "null handle");
         if (!d_llr) fail(Kind::Config, "null LLR buffer");
-        TrialGenArgs a = generator_args(h->code, current_device(), modulation, snr_kind, snr_db, seed, point);
+        TrialGenArgs a = make_generator_args(h->code, current_device(), modulation, snr_kind, snr_db, seed, point);
         a.trial0 = trial0;
         a.count = int64_t(count);
         a.llr = d_llr;

@@ -28,17 +28,12 @@ void check_hip(hipError_t err, const char* what) {
     if (err != hipSuccess) fail(Kind::Device, std::string(what) + ": " + hipGetErrorString(err));
 }
 
-struct WorkspaceSlot {
-    std::mutex mu;
-    Workspace ws;
-};
-
 std::mutex g_ws_mu;
-std::map<std::pair<int, hipStream_t>, std::unique_ptr<WorkspaceSlot>> g_ws;
+std::map<std::pair<std::pair<int, int>, hipStream_t>, std::unique_ptr<WorkspaceSlot>> g_ws;
 
-WorkspaceSlot& workspace(int device, hipStream_t s) {
+WorkspaceSlot& workspace(int device, hipStream_t s, int purpose) {
     std::lock_guard<std::mutex> l(g_ws_mu);
-    auto& slot = g_ws[{device, s}];
+    auto& slot = g_ws[{{device, purpose}, s}];
     if (!slot) slot = std::make_unique<WorkspaceSlot>();
     return *slot;
 }
@@ -84,7 +79,7 @@ GenericArgs<T> generic_args(const cf_code* h, int device) {
 
 template <typename T>
 void run_generic(const GenericArgs<T>& a, int device, hipStream_t stream) {
-    WorkspaceSlot& slot = workspace(device, stream);
+    WorkspaceSlot& slot = workspace(device, stream, 0);
     std::lock_guard<std::mutex> l(slot.mu);
     check_hip(launch_generic(a, device, stream, slot.ws), "decode launch");
 }

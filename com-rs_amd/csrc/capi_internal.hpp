@@ -24,6 +24,17 @@ namespace comfec {
 namespace detail {
 
 extern thread_local std::string g_error;
+
+// A grow-only device buffer per (device, stream, purpose); hold `mu` while using it.
+struct WorkspaceSlot {
+    std::mutex mu;
+    Workspace ws;
+};
+WorkspaceSlot& workspace(int device, hipStream_t s, int purpose);
+DevCode dev_code(const NrCode& c, int device);
+// the generator's arguments for a code, a modulation and one SNR (campaign.cpp)
+TrialGenArgs make_generator_args(const NrCode& c, int device, int modulation, int snr_kind, double snr_db,
+                                 uint64_t seed, uint32_t point);
 cf_status status_of(Kind k);
 void check_hip(hipError_t err, const char* what);
 int current_device();

@@ -209,8 +209,14 @@ __global__ void __launch_bounds__(384) decode_layered_kernel(LayeredArgs a) {
     uint32_t* hd = reinterpret_cast<uint32_t*>(lds + n);
     const int r = threadIdx.x;
     const bool active = r < z;
-    const int64_t w = blockIdx.x;
-    if (w >= a.count) return;
+    int64_t w = blockIdx.x;
+    if (a.index) {
+        // a subset: the listed codewords (the count lives on the device)
+        if (uint32_t(w) >= *a.index_count) return;
+        w = a.index[w];
+    } else if (w >= a.count) {
+        return;
+    }
 
     // channel LLRs of the full codeword (nr/ldpc.rs:191-197): repeats summed in output
     // order, 0.0 on unsent bits, 30.0 on fillers
@@ -236,8 +242,9 @@ __global__ void __launch_bounds__(384) decode_layered_kernel(LayeredArgs a) {
         __syncthreads();
         if (!syndrome_fails<BG>(post, hd, a)) done = it;
     }
+    if (a.ok && threadIdx.x == 0) a.ok[w] = done >= 0;
     if (done < 0) done = a.max_iterations;
-    if (threadIdx.x == 0 && a.iterations) a.iterations[w] = uint32_t(done);
+    if (threadIdx.x == 0 && a.iterations) a.iterations[w] = uint32_t(done) + (a.add_iterations ? a.iterations[w] : 0u);
     if (a.bits)
         for (int p = threadIdx.x; p < a.k_info; p += blockDim.x) a.bits[w * a.k_info + p] = post[p] < 0.0f;
     if (a.post_out)

@@ -56,9 +56,38 @@ struct LayeredArgs {
     uint8_t* bits;
     float* post_out;
     uint32_t* iterations;
+    uint8_t* ok;                // per codeword: the syndrome holds at the end, or null
+    const uint32_t* index;      // decode codewords index[0 .. *index_count) only, or null
+    const uint32_t* index_count;
+    int add_iterations;         // add to iterations[] instead of storing
 };
 
 bool layered_supported(int bg, int z);
+// the codewords whose syndrome failed: out[0 .. *n) (n zeroed by the caller's fill kernel)
+hipError_t launch_failed_list(const uint8_t* ok, int64_t count, uint32_t* out, uint32_t* n, hipStream_t stream);
+
+// Training of the learned offset/scale min-sum (T unrolled layered iterations).
+struct LearnArgs {
+    DevCode code;
+    int T;
+    const float* params;     // (a, b) per iteration and base row
+    const float* llr;        // count x e
+    const uint32_t* cw;      // count x ceil(n/32) codeword bits
+    int64_t count;
+    float scale;             // 1 / (T n count)
+    float* ckpt;             // count x T x edges: variable-to-check messages
+    float* gpost;            // count x T x n: loss gradient wrt the posteriors of each iteration
+    float* cbar;             // count x edges: backward scratch
+    float* loss_out;         // count
+    float* grad_partial;     // count x (T rows 2)
+    float* grad;             // T rows 2
+    float* loss;             // 1
+};
+
+hipError_t launch_learn_forward(const LearnArgs& a, hipStream_t stream);
+hipError_t launch_learn_backward(const LearnArgs& a, hipStream_t stream);
+hipError_t launch_adam(float* params, const float* grad, float* m, float* v, int n, float lr, float b1, float b2,
+                       float eps, int step, hipStream_t stream);
 hipError_t launch_layered(const LayeredArgs& a, int bg, hipStream_t stream);
 
 // The device trial generator: information bits, encoder, rate matching, channel, LLRs.
@@ -80,6 +109,10 @@ struct TrialGenArgs {
     int64_t count;
     float* llr;                // count x e
     uint32_t* info;            // count x ceil(K'/32), LSB first
+    uint32_t* codeword;        // count x ceil(N/32) codeword bits, or null
+    float snr_lo, snr_hi;      // per-trial SNR uniform in [lo, hi] dB when hi > lo
+    int snr_kind;              // 0 Eb/N0, 1 Es/N0 (per-trial SNR)
+    float rate;                // K'/E (per-trial SNR)
 };
 
 // The error counts of a decoded batch.

@@ -90,8 +90,29 @@ __global__ void __launch_bounds__(kGenThreads) generate_kernel(TrialGenArgs a) {
     }
     __syncthreads();
 
+    if (a.codeword)
+        for (int wi = threadIdx.x; wi < (a.n + 31) / 32; wi += blockDim.x) {
+            uint32_t word = 0;
+            for (int b = 0; b < 32 && wi * 32 + b < a.n; ++b) word |= uint32_t(cw[wi * 32 + b]) << b;
+            a.codeword[w * ((a.n + 31) / 32) + wi] = word;
+        }
+
     // modulation, noise and LLRs of the E transmitted bits
     const int bps = a.bits_per_symbol;
+    float noise_var = a.noise_var, noise_sd = a.noise_sd;
+    if (a.snr_hi > a.snr_lo) {
+        // this trial's SNR, uniform in [lo, hi] dB (Philox purpose 2)
+        const U4 u = philox4x32_10(trial_counter(a.point, trial, 2, 0u), a.k0, a.k1);
+        const float snr = a.snr_lo + (a.snr_hi - a.snr_lo) * (float(u.x >> 8) * (1.0f / 16777216.0f));
+        const float lin = exp10f(snr / 10.0f);
+        if (bps == 1) {
+            noise_var = a.snr_kind == 0 ? 1.0f / (2.0f * a.rate * lin) : 1.0f / (2.0f * lin);
+            noise_sd = sqrtf(noise_var);
+        } else {
+            noise_var = a.snr_kind == 0 ? 1.0f / (float(bps) * a.rate * lin) : 1.0f / lin;
+            noise_sd = sqrtf(noise_var / 2.0f);
+        }
+    }
     const int nsym = a.e / bps;
     float* out = a.llr + w * a.e;
     for (int s = threadIdx.x; s < nsym; s += blockDim.x) {
@@ -103,20 +124,20 @@ __global__ void __launch_bounds__(kGenThreads) generate_kernel(TrialGenArgs a) {
         const float g0 = rad * cs, g1 = rad * sn;
         if (bps == 1) {
             const float x = 1.0f - 2.0f * float(cw[a.map[s]]);
-            const float y = x + a.noise_sd * g0;
-            out[s] = 2.0f * y / a.noise_var;
+            const float y = x + noise_sd * g0;
+            out[s] = 2.0f * y / noise_var;
             continue;
         }
         int label = 0;
         for (int l = 0; l < bps; ++l) label = (label << 1) | cw[a.map[s * bps + l]];
-        const float yr = a.pts_re[label] + a.noise_sd * g0;
-        const float yi = a.pts_im[label] + a.noise_sd * g1;
+        const float yr = a.pts_re[label] + noise_sd * g0;
+        const float yi = a.pts_im[label] + noise_sd * g1;
         const int m = 1 << bps;
         for (int l = 0; l < bps; ++l) {
             float best0 = -INFINITY, best1 = -INFINITY;
             for (int i = 0; i < m; ++i) {
                 const float dr = yr - a.pts_re[i], di = yi - a.pts_im[i];
-                const float metric = -(dr * dr + di * di) / a.noise_var + a.log_prior;
+                const float metric = -(dr * dr + di * di) / noise_var + a.log_prior;
                 if ((i >> (bps - 1 - l)) & 1) best1 = fmaxf(best1, metric);
                 else best0 = fmaxf(best0, metric);
             }
@@ -154,6 +175,11 @@ __global__ void __launch_bounds__(256) count_kernel(CountArgs a) {
     }
 }
 
+__global__ void failed_list_kernel(const uint8_t* ok, int64_t count, uint32_t* out, uint32_t* n) {
+    const int64_t w = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (w < count && !ok[w]) out[atomicAdd(n, 1u)] = uint32_t(w);
+}
+
 __global__ void zero_kernel(uint64_t* p, int n) {
     if (int(threadIdx.x) < n) p[threadIdx.x] = 0;
 }
@@ -162,6 +188,13 @@ __global__ void zero_kernel(uint64_t* p, int n) {
 
 hipError_t launch_zero(uint64_t* p, int n, hipStream_t stream) {
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_failed_list(const uint8_t* ok, int64_t count, uint32_t* out, uint32_t* n, hipStream_t stream) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(failed_list_kernel, dim3(unsigned((count + 255) / 256)), dim3(256), 0, stream, ok, count, out,
+                       n);
     return hipGetLastError();
 }
 

@@ -215,6 +215,44 @@ cf_status cf_results_metric(const cf_results* res, size_t i, const char* name, d
                             cf_metric* out);
 void cf_results_free(cf_results* res);
 
+/* ---- the learned offset/scale min-sum (no reference counterpart) ---------------------------
+   The layered min-sum of ldpc.rs:509-531 whose check-node magnitude map, at iteration t and
+   base row i, is g(m) = a[t][i] max(m - b[t][i], 0); the parameters are float32 pairs (a, b)
+   laid out [t][i][2] in device memory (a = alpha, b = 0 is the normalized min-sum). Training
+   minimises the mean binary cross-entropy of sigmoid(-post) against the codeword bits over
+   the unrolled iterations. */
+
+/* Training data from the device trial generator: as cf_generate_batch, with each trial's
+   SNR uniform in [snr_lo_db, snr_hi_db], and the count x ceil(N/32) packed codeword bits. */
+cf_status cf_generate_training_batch(const cf_code* code, int modulation, int snr_kind,
+                                     double snr_lo_db, double snr_hi_db, uint64_t seed,
+                                     uint32_t point, uint64_t trial0, size_t count, float* d_llr,
+                                     uint32_t* d_codeword, void* stream);
+
+/* The loss over `count` codewords and its gradient with respect to the parameters
+   (d_grad: iterations x rows x 2 floats), by the forward and backward kernels. */
+cf_status cf_learned_loss_grad(const cf_code* code, uint32_t iterations, const float* d_params,
+                               const float* d_llr, const uint32_t* d_codeword, size_t count,
+                               float* d_grad, float* d_loss, void* stream);
+
+/* One Adam step (Kingma & Ba, 2015) on n parameters; step counts from 1. */
+cf_status cf_adam_step(float* d_params, const float* d_grad, float* d_m, float* d_v, size_t n,
+                       uint32_t step, double lr, double beta1, double beta2, double eps, void* stream);
+
+/* `steps` Adam steps (lr, beta 0.9 / 0.999) on batches of the device generator (trial t of
+   step s is trial s*batch + t of grid point 0); the loss of every step to loss_history_host. */
+cf_status cf_learned_train(const cf_code* code, uint32_t iterations, float* d_params, uint32_t steps,
+                           uint32_t batch, double lr, int modulation, int snr_kind, double snr_lo_db,
+                           double snr_hi_db, uint64_t seed, float* loss_history_host, void* stream);
+
+/* Decode with the learned parameters (layered, syndrome stop). With fallback set, every
+   codeword whose syndrome still fails is decoded again from its channel LLRs by the code's
+   own min-sum settings, and its iteration count is the sum of both. d_ok: per codeword,
+   whether the learned decoder alone ended on a codeword. */
+cf_status cf_learned_decode_batch(const cf_code* code, uint32_t iterations, const float* d_params,
+                                  int fallback, const float* d_llr, size_t count, uint8_t* d_bits,
+                                  uint32_t* d_iterations, uint8_t* d_ok, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

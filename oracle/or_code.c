@@ -10,14 +10,7 @@
 #include "oracle.h"
 #include "../com-rs_amd/csrc/nr_base_graphs.inc"
 
-struct or_code {
-    int32_t n, m, k, k_info, e, bg, z, ils, core;
-    int32_t* row_start; /* m + 1 */
-    int32_t* cols;      /* edges */
-    int32_t* solve_start; /* core + 1: rows of the inverse of the core block */
-    int32_t* solve;
-    int32_t* map;       /* e output positions (NR codes) */
-};
+#include "or_internal.h"
 
 static __thread char g_err[256];
 const char* or_last_error(void) { return g_err; }
