@@ -103,6 +103,7 @@ void run_layered(const cf_code* h, int device, const float* llr, int64_t count, 
     a.row_start = t.row_start;
     a.col = t.col;
     a.shift = t.shift;
+    a.shift32 = t.shift32;
     a.src_start = t.src_start;
     a.src = t.src;
     // g(m) = a max(m - b, 0): (alpha, 0) normalized, (1, beta) offset min-sum

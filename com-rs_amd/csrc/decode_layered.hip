@@ -96,7 +96,7 @@ __device__ inline float with_sign(float mag, uint32_t bit) {
 
 // ldpc.rs:522-531 for the row of lane r in layer I
 template <int BG, int I>
-__device__ __forceinline__ void layer(float* __restrict__ post, int r, int z, const uint16_t* __restrict__ shift,
+__device__ __forceinline__ void layer(float* __restrict__ post, int r, int z, const uint32_t* __restrict__ shift,
                                       Messages<Graph<BG>::rows>& msg, float a, float b) {
     constexpr int e0 = row_begin<BG>(I);
     constexpr int deg = row_begin<BG>(I + 1) - e0;
@@ -110,7 +110,7 @@ __device__ __forceinline__ void layer(float* __restrict__ post, int r, int z, co
     uint32_t neg = 0;
 #pragma unroll
     for (int q = 0; q < deg; ++q) {
-        addr[q] = int(Graph<BG>::t[e0 + q].j) * z + wrap_add(r, shift[e0 + q], z);
+        addr[q] = int(Graph<BG>::t[e0 + q].j) * z + wrap_add(r, int(shift[e0 + q]), z);
         const float old = with_sign(q == i1o ? g2o : g1o, (pko >> q) & 1u);
         const float in = post[addr[q]] - old;
         v2c[q] = in;
@@ -137,7 +137,7 @@ __device__ __forceinline__ void layer(float* __restrict__ post, int r, int z, co
 }
 
 template <int BG, int I>
-__device__ __forceinline__ void sweep(float* post, int r, int z, bool active, const uint16_t* shift,
+__device__ __forceinline__ void sweep(float* post, int r, int z, bool active, const uint32_t* shift,
                                       Messages<Graph<BG>::rows>& msg, const float* ab, float a, float b) {
     if constexpr (I < Graph<BG>::rows) {
         if (ab) {
@@ -151,8 +151,14 @@ __device__ __forceinline__ void sweep(float* post, int r, int z, bool active, co
 }
 
 // whether a check fails (ldpc.rs:555-557), packed path: Z a multiple of 32
+struct Tables {
+    const uint32_t* shift;
+    const uint16_t* col;
+    const uint16_t* row_start;
+};
+
 template <int BG>
-__device__ bool syndrome_fails_packed(const float* post, uint32_t* hd, const LayeredArgs& a) {
+__device__ bool syndrome_fails_packed(const float* post, uint32_t* hd, const LayeredArgs& a, const Tables& tb) {
     const int z = a.z, w32 = z >> 5, n = Graph<BG>::cols * z;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
     for (int c = wave; c < (n >> 6); c += waves) {
@@ -170,9 +176,9 @@ __device__ bool syndrome_fails_packed(const float* post, uint32_t* hd, const Lay
     for (int task = threadIdx.x; task < Graph<BG>::rows * w32; task += blockDim.x) {
         const int i = task / w32, w = task - i * w32;
         uint32_t par = 0;
-        for (int eb = a.row_start[i]; eb < a.row_start[i + 1]; ++eb) {
-            const int v = a.shift[eb];
-            const int base = a.col[eb] * 2 * w32 + w + (v >> 5);
+        for (int eb = tb.row_start[i]; eb < tb.row_start[i + 1]; ++eb) {
+            const int v = int(tb.shift[eb]);
+            const int base = tb.col[eb] * 2 * w32 + w + (v >> 5);
             par ^= __builtin_amdgcn_alignbit(hd[base + 1], hd[base], uint32_t(v & 31));
         }
         bad |= par;
@@ -182,23 +188,24 @@ __device__ bool syndrome_fails_packed(const float* post, uint32_t* hd, const Lay
 
 // general Z: every lane checks its rows edge by edge
 template <int BG>
-__device__ bool syndrome_fails_rows(const float* post, const LayeredArgs& a) {
+__device__ bool syndrome_fails_rows(const float* post, const LayeredArgs& a, const Tables& tb) {
     const int z = a.z;
     int bad = 0;
     for (int r = threadIdx.x; r < z; r += blockDim.x)
         for (int i = 0; i < Graph<BG>::rows && !bad; ++i) {
             int par = 0;
-            for (int eb = a.row_start[i]; eb < a.row_start[i + 1]; ++eb)
-                par ^= post[a.col[eb] * z + wrap_add(r, a.shift[eb], z)] < 0.0f;
+            for (int eb = tb.row_start[i]; eb < tb.row_start[i + 1]; ++eb)
+                par ^= post[tb.col[eb] * z + wrap_add(r, int(tb.shift[eb]), z)] < 0.0f;
             bad |= par;
         }
     return __syncthreads_or(bad) != 0;
 }
 
 template <int BG>
-__device__ __forceinline__ bool syndrome_fails(const float* post, uint32_t* hd, const LayeredArgs& a) {
-    if ((a.z & 31) == 0) return syndrome_fails_packed<BG>(post, hd, a);
-    return syndrome_fails_rows<BG>(post, a);
+__device__ __forceinline__ bool syndrome_fails(const float* post, uint32_t* hd, const LayeredArgs& a,
+                                               const Tables& tb) {
+    if ((a.z & 31) == 0) return syndrome_fails_packed<BG>(post, hd, a, tb);
+    return syndrome_fails_rows<BG>(post, a, tb);
 }
 
 template <int BG>
@@ -218,6 +225,17 @@ __global__ void __launch_bounds__(384) decode_layered_kernel(LayeredArgs a) {
         return;
     }
 
+    // the shifts of the layers (read with uniform addresses) and, for the syndrome, the
+    // columns and row starts, in LDS
+    __shared__ uint32_t shift_lds[Graph<BG>::edges];
+    __shared__ uint16_t col_lds[Graph<BG>::edges];
+    __shared__ uint16_t rs_lds[Graph<BG>::rows + 1];
+    for (int e = threadIdx.x; e < Graph<BG>::edges; e += blockDim.x) {
+        shift_lds[e] = a.shift32[e];
+        col_lds[e] = a.col[e];
+    }
+    for (int i = threadIdx.x; i <= Graph<BG>::rows; i += blockDim.x) rs_lds[i] = a.row_start[i];
+
     // channel LLRs of the full codeword (nr/ldpc.rs:191-197): repeats summed in output
     // order, 0.0 on unsent bits, 30.0 on fillers
     for (int p = threadIdx.x; p < n; p += blockDim.x) {
@@ -235,12 +253,13 @@ __global__ void __launch_bounds__(384) decode_layered_kernel(LayeredArgs a) {
         msg.g2[i] = 0.0f;
         msg.pk[i] = 0u;
     }
-    int done = syndrome_fails<BG>(post, hd, a) ? -1 : 0;
+    const Tables tb{shift_lds, col_lds, rs_lds};
+    int done = syndrome_fails<BG>(post, hd, a, tb) ? -1 : 0;
     for (int it = 1; it <= a.max_iterations && done < 0; ++it) {
         const float* ab = a.params ? a.params + size_t(it - 1) * 2 * Graph<BG>::rows : nullptr;
-        sweep<BG, 0>(post, r, z, active, a.shift, msg, ab, a.a, a.b);
+        sweep<BG, 0>(post, r, z, active, shift_lds, msg, ab, a.a, a.b);
         __syncthreads();
-        if (!syndrome_fails<BG>(post, hd, a)) done = it;
+        if (!syndrome_fails<BG>(post, hd, a, tb)) done = it;
     }
     if (a.ok && threadIdx.x == 0) a.ok[w] = done >= 0;
     if (done < 0) done = a.max_iterations;

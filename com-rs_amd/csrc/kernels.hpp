@@ -46,6 +46,7 @@ struct LayeredArgs {
     const uint16_t* row_start;  // base tables (device)
     const uint16_t* col;
     const uint16_t* shift;
+    const uint32_t* shift32;    // the same shifts as 32-bit words
     const uint32_t* src_start;
     const uint32_t* src;
     float a, b;                 // g(m) = a * max(m - b, 0)

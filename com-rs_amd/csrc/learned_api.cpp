@@ -146,6 +146,7 @@ cf_status cf_learned_decode_batch(const cf_code* h, uint32_t iterations, const f
         a.row_start = t.row_start;
         a.col = t.col;
         a.shift = t.shift;
+        a.shift32 = t.shift32;
         a.src_start = t.src_start;
         a.src = t.src;
         a.params = d_params;
@@ -184,15 +185,19 @@ cf_status cf_learned_train(const cf_code* h, uint32_t iterations, float* d_param
         const size_t np = size_t(iterations) * c.rows_b * 2;
         const size_t words = (c.n + 31) / 32;
         float* buf = nullptr;
-        const size_t floats = size_t(batch) * c.e + size_t(batch) * words + 3 * np + steps;
-        check_hip(hipMalloc(&buf, floats * sizeof(float)), "hipMalloc");
+        // regions at multiples of 64 floats (256 bytes), so the u64 fills stay aligned
+        auto up = [](size_t x) { return (x + 63) & ~size_t(63); };
+        const size_t at_cw = up(size_t(batch) * c.e), at_grad = at_cw + up(size_t(batch) * words);
+        const size_t at_m = at_grad + up(np), at_losses = at_m + up(2 * np);
+        check_hip(hipMalloc(&buf, (at_losses + up(steps)) * sizeof(float)), "hipMalloc");
         std::unique_ptr<float, void (*)(float*)> guard(buf, [](float* p) { (void)hipFree(p); });
         float* llr = buf;
-        uint32_t* cw = reinterpret_cast<uint32_t*>(llr + size_t(batch) * c.e);
-        float* grad = reinterpret_cast<float*>(cw + size_t(batch) * words);
-        float* m = grad + np;
+        uint32_t* cw = reinterpret_cast<uint32_t*>(buf + at_cw);
+        float* grad = buf + at_grad;
+        float* m = buf + at_m;
         float* v = m + np;
-        float* losses = v + np;
+        float* losses = buf + at_losses;
+        // the Adam moments m and v (2 np floats, contiguous) start at zero
         check_hip(launch_zero(reinterpret_cast<uint64_t*>(m), int(np), s), "zero launch");
         for (uint32_t step = 1; step <= steps; ++step) {
             if (cf_generate_training_batch(h, modulation, snr_kind, snr_lo_db, snr_hi_db, seed, 0,

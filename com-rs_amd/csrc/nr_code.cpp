@@ -252,7 +252,7 @@ const NrCode::DeviceTables& NrCode::device_tables(int device) const {
     // one buffer: u16 tables, then u32 tables (4-byte aligned)
     size_t n16 = row_start.size() + col.size() + shift.size() + col_start.size() + col_edge.size();
     n16 = (n16 + 1) & ~size_t(1);
-    size_t n32 = src_start.size() + src.size() + map.size();
+    size_t n32 = src_start.size() + src.size() + map.size() + shift.size();
     n32 = (n32 + 1) & ~size_t(1);
     size_t bytes = n16 * 2 + n32 * 4 + core_inv.size() * 8;
     std::vector<uint8_t> host(bytes, 0);
@@ -270,6 +270,8 @@ const NrCode::DeviceTables& NrCode::device_tables(int device) const {
     std::memcpy(h32, src_start.data(), src_start.size() * 4);
     std::memcpy(h32 + src_start.size(), src.data(), src.size() * 4);
     std::memcpy(h32 + src_start.size() + src.size(), map.data(), map.size() * 4);
+    uint32_t* h_shift = h32 + src_start.size() + src.size() + map.size();
+    for (size_t i = 0; i < shift.size(); ++i) h_shift[i] = shift[i];
     uint64_t* h64 = reinterpret_cast<uint64_t*>(host.data() + n16 * 2 + n32 * 4);
     std::memcpy(h64, core_inv.data(), core_inv.size() * 8);
     int prev = 0;
@@ -294,6 +296,7 @@ const NrCode::DeviceTables& NrCode::device_tables(int device) const {
     t.src_start = d32;
     t.src = d32 + src_start.size();
     t.map = d32 + src_start.size() + src.size();
+    t.shift32 = d32 + src_start.size() + src.size() + map.size();
     t.core_inv = reinterpret_cast<const uint64_t*>(static_cast<const uint8_t*>(base) + n16 * 2 + n32 * 4);
     return t;
 }

@@ -81,6 +81,7 @@ struct NrCode {
         const uint32_t* src = nullptr;
         const uint32_t* map = nullptr;
         const uint64_t* core_inv = nullptr;
+        const uint32_t* shift32 = nullptr;  // the shifts as 32-bit words (scalar loads)
     };
     const DeviceTables& device_tables(int device) const;
     ~NrCode();

@@ -181,13 +181,15 @@ __global__ void failed_list_kernel(const uint8_t* ok, int64_t count, uint32_t* o
 }
 
 __global__ void zero_kernel(uint64_t* p, int n) {
-    if (int(threadIdx.x) < n) p[threadIdx.x] = 0;
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = 0;
 }
 
 }  // namespace
 
 hipError_t launch_zero(uint64_t* p, int n, hipStream_t stream) {
-    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, stream, p, n);
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, p, n);
     return hipGetLastError();
 }
 

@@ -73,24 +73,33 @@ def test_training_lowers_the_loss_and_beats_the_plain_rule():
     code = F.NrLdpc.lifted(2, 52).set_decoder("normalized_min_sum", 0.75, True, 12)
     T = 12
     params = F.initial_params(code, T)
-    hist = F.learned_train(code, params, steps=300, batch=2048, lr=3e-3, modulation=F.BPSK,
+    # a fixed validation batch (its own seed): the loss before and after training
+    vllr, vcw = F.generate_training_batch(code, 2048, F.BPSK, 0.0, 3.0, seed=99)
+    before, _ = F.learned_loss_grad(code, params, vllr, vcw)
+    hist = F.learned_train(code, params, steps=400, batch=4096, lr=1e-2, modulation=F.BPSK,
                            snr_lo=0.0, snr_hi=3.0, seed=1)
     assert np.all(np.isfinite(hist))
-    assert hist[-30:].mean() < 0.97 * hist[:30].mean()
-    # evaluation on fresh trials (another seed) at 2 dB: learned vs plain, same iterations
-    llr, info = F.generate_batch(code, 40000, F.BPSK, 2.0, seed=12345)
+    after, _ = F.learned_loss_grad(code, params, vllr, vcw)
+    print(f"validation loss {before.item():.5f} -> {after.item():.5f}", flush=True)
+    assert after.item() < before.item()
+    # evaluation on fresh trials at 1.5 dB: learned vs plain, the same 12 iterations
+    n = 40000
+    llr, info = F.generate_batch(code, n, F.BPSK, 1.5, seed=12345)
     plain_bits, plain_its, _ = code.decode_batch(llr)
     bits, its, ok = F.learned_decode_batch(code, params, llr, fallback=False)
     f_plain, f_learned = fer(plain_bits, info), fer(bits, info)
     print(f"FER plain {f_plain:.4g} learned {f_learned:.4g}; iterations "
           f"{plain_its.float().mean().item():.3f} vs {its.float().mean().item():.3f}", flush=True)
-    n = 40000
     lo_p, hi_p = O.wilson(int(round(f_plain * n)), n, 0.95)
     lo_l, hi_l = O.wilson(int(round(f_learned * n)), n, 0.95)
-    assert hi_l < lo_p or f_learned < f_plain
-    # the fallback re-decodes the frames whose syndrome failed: never more errors
+    assert hi_l < lo_p
+    # the fallback decodes again, with the plain rule, the frames whose syndrome failed
     bits_fb, its_fb, ok_fb = F.learned_decode_batch(code, params, llr, fallback=True)
     assert torch.equal(ok, ok_fb)
-    assert fer(bits_fb, info) <= f_learned
-    failed = ok.cpu().numpy() == 0
+    okn = ok.cpu().numpy().astype(bool)
+    assert torch.equal(bits_fb[torch.from_numpy(okn).cuda()], bits[torch.from_numpy(okn).cuda()])
+    failed = ~okn
     assert np.all(its_fb.cpu().numpy()[failed] >= T)
+    # the frames the fallback decoded are the plain decoder's own decisions
+    pb = plain_bits.cpu().numpy()[failed]
+    assert np.array_equal(bits_fb.cpu().numpy()[failed], pb)
