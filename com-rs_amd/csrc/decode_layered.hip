@@ -276,8 +276,10 @@ hipError_t launch_bg(const LayeredArgs& a, hipStream_t stream) {
     const int n = Graph<BG>::cols * a.z;
     const size_t lds = size_t(n) * 4 + size_t(Graph<BG>::cols) * 2 * ((a.z + 31) / 32) * 4;
     if (lds > kLdsBudget + 8 * 1024) return hipErrorInvalidValue;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(decode_layered_kernel<BG>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(kLdsBudget + 8 * 1024));
+    // the dynamic LDS of this launch (the static tables come on top of it)
+    hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(decode_layered_kernel<BG>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (err != hipSuccess) return err;
     hipLaunchKernelGGL(decode_layered_kernel<BG>, dim3(unsigned(a.count)), dim3(threads), lds, stream, a);
     return hipGetLastError();
 }

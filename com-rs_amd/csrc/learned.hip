@@ -276,8 +276,9 @@ hipError_t launch_learn_forward(const LearnArgs& a, hipStream_t stream) {
     if (a.count <= 0) return hipSuccess;
     const int threads = ((a.code.z + 63) / 64) * 64;
     const size_t lds = size_t(a.code.n) * 4;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(learn_forward_kernel),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(kLdsBudget + 8 * 1024));
+    hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(learn_forward_kernel),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (err != hipSuccess) return err;
     hipLaunchKernelGGL(learn_forward_kernel, dim3(unsigned(a.count)), dim3(threads), lds, stream, a);
     return hipGetLastError();
 }
@@ -288,10 +289,11 @@ hipError_t launch_learn_backward(const LearnArgs& a, hipStream_t stream) {
     const size_t np = size_t(a.T) * a.code.rows_b * 2;
     const size_t lds = size_t(a.code.n) * 4 + size_t(threads / 64) * np * 4;
     if (lds > kLdsBudget + 8 * 1024) return hipErrorInvalidValue;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(learn_backward_kernel),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(kLdsBudget + 8 * 1024));
+    hipError_t err = hipFuncSetAttribute(reinterpret_cast<const void*>(learn_backward_kernel),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (err != hipSuccess) return err;
     hipLaunchKernelGGL(learn_backward_kernel, dim3(unsigned(a.count)), dim3(threads), lds, stream, a);
-    hipError_t err = hipGetLastError();
+    err = hipGetLastError();
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(reduce_kernel, dim3(unsigned(np + 1)), dim3(256), 0, stream, a.grad_partial, a.loss_out,
                        a.count, int(np), a.grad, a.loss);

@@ -169,6 +169,7 @@ cf_status cf_learned_decode_batch(const cf_code* h, uint32_t iterations, const f
         a.index = list;
         a.index_count = n_fail;
         a.add_iterations = 1;
+        a.ok = nullptr;  // ok keeps the learned decoder's own verdict
         check_hip(launch_layered(a, c.bg, s), "fallback decode launch");
     });
 }
