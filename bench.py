@@ -1,12 +1,18 @@
-"""Throughput of the NR LDPC decoder on one MI355X.
+"""Throughput of the NR LDPC decoders on one MI355X at the FER 1e-3 operating point:
+the learned offset/scale min-sum against the plain normalized min-sum.
 
-Workload: NR base graph 1, Z=384 (K'=8448, E=25344, rate 1/3), Gray 16-QAM over AWGN,
-max-log LLRs, layered normalized min-sum (alpha 0.75), 10 iterations with syndrome early
-stop, FP32 messages, 4096 codewords per step. The Es/N0 of the timed steps is the point of
-the grid {4, 5, 6, 7} dB whose FER is nearest 1e-3 (measured on the first batch).
-A step decodes one batch of 4096 codewords whose LLRs are already resident in HBM.
+Workload: NR base graph 1, Z=384 (K'=8448, E=25344, rate 1/3), Gray 16-QAM over AWGN with
+max-log LLRs and the NR bit interleaver, 4096 codewords per step, FP32 messages, layered
+schedule with syndrome early stop. The plain decoder is the normalized min-sum (alpha 0.75)
+with 10 iterations. The operating point is the Es/N0 at which the plain decoder's FER is
+nearest 1e-3 (measured on a grid of 0.1 dB over 16384 frames per point). The learned
+decoder (per-iteration, per-base-row offset and scale, at most LEARNED_ITERATIONS
+iterations, codewords whose syndrome still fails decoded again by the plain decoder) is
+trained on the GPU before the timed steps, on LLRs of the device generator around that
+point. A step decodes one batch whose LLRs are already resident in HBM; the timed steps
+decode the same batch with each decoder.
 
-Prints one JSON line (see the contract in the repository root docs).
+Prints one JSON line; `value` is the learned decoder's decoded Gbit/s.
 """
 import argparse
 import json
@@ -20,6 +26,8 @@ sys.path.insert(0, os.path.join(ROOT, "com-rs_amd"))
 
 HBM_PEAK_GBS = 8000.0
 LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz
+LEARNED_ITERATIONS = 8
+K_INFO, E, Z, BG1_EDGES = 8448, 25344, 384, 316
 
 
 def parse():
@@ -28,21 +36,20 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--batch", type=int, default=4096)
-    p.add_argument("--iterations", type=int, default=10)
+    p.add_argument("--train-steps", type=int, default=1000)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     return p.parse_args()
 
 
-def cpu_baseline(code_args, llr_f32, seconds):
-    """The oracle's f64 decoder (a C restatement of LdpcCode::propagate) on host threads,
-    over a bounded sample of the same LLRs: decoded Gbit/s."""
+def cpu_baseline(llr_f32, seconds):
+    """The oracle's f64 layered normalized min-sum (a C restatement of LdpcCode::propagate,
+    10 iterations, syndrome stop) on host threads over a bounded sample of the timed batch."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    ref = O.Code.nr(*code_args)
+    ref = O.Code.nr(K_INFO, E, 1, 0, 4)
     threads = max(1, min(16, os.cpu_count() or 1))
-    # a first small run sizes the sample to about `seconds` of CPU work
     probe = min(threads, llr_f32.shape[0])
     fulls = np.array([ref.full_llr(l.astype(np.float64)) for l in llr_f32[:probe]])
     t0 = time.perf_counter()
@@ -53,9 +60,29 @@ def cpu_baseline(code_args, llr_f32, seconds):
     t0 = time.perf_counter()
     ref.decode_many(fulls, rule=O.NMS, alpha=0.75, layered=True, max_iterations=10, threads=threads)
     dt = time.perf_counter() - t0
-    return {"value": ref.k_info * count / dt / 1e9, "unit": "Gbit/s", "cores": threads, "kind": "port",
-            "sample": f"{count} codewords of the timed batch (f64 layered NMS 0.75, 10 it, "
-                      f"oracle C restatement, {threads} threads, {dt:.1f} s)"}
+    return {"value": round(K_INFO * count / dt / 1e9, 6), "unit": "Gbit/s", "cores": threads, "kind": "port",
+            "sample": f"{count} codewords of the timed batch: f64 layered NMS 0.75, 10 it, syndrome stop "
+                      f"(oracle C restatement of LdpcCode::propagate), {threads} threads, {dt:.1f} s"}
+
+
+def frame_errors(bits, info, F):
+    import numpy as np
+    return int((bits.cpu().numpy() != F.unpack_info(info, bits.shape[1])).any(1).sum())
+
+
+def timed(fn, steps, warmup, torch):
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record(stream)
+    for _ in range(steps):
+        fn()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps, time.perf_counter() - t0
 
 
 def main():
@@ -67,81 +94,116 @@ def main():
     import numpy as np
     import torch
     import comfec as F
-    import workloads as W
 
     torch.cuda.init()
-    dev = torch.device("cuda:0")
-    k_info, e = 8448, 25344
-    code = F.NrLdpc(k_info, e, 1, 0, 4).set_decoder("normalized_min_sum", 0.75, True, args.iterations)
+    code = F.NrLdpc(K_INFO, E, 1, 0, 4).set_decoder("normalized_min_sum", 0.75, True, 10)
     batch = args.batch
 
-    # calibration: FER at every grid point, on the batch that the timed steps will use
-    points = {}
-    for esn0 in (4.0, 5.0, 6.0, 7.0):
-        info, llr = W.nr_qam_batch(code, batch, esn0, m=16, seed=1)
+    # the operating point: plain FER nearest 1e-3 on a 0.1 dB grid (16384 frames per point)
+    calib = {}
+    for i in range(8):
+        esn0 = round(3.9 + 0.1 * i, 2)
+        fe, n = 0, 0
+        for part in range(4):
+            llr, info = F.generate_batch(code, 4096, F.QAM16, esn0, snr_kind=F.ESN0_DB, seed=2, point=i,
+                                         trial0=4096 * part)
+            bits, _, _ = code.decode_batch(llr)
+            fe += frame_errors(bits, info, F)
+            n += 4096
+        calib[esn0] = (fe, n)
+    esn0 = min(calib, key=lambda p: abs(math.log10(max(calib[p][0], 0.5) / calib[p][1]) + 3.0))
+    # the config grid of the workload, for the record
+    grid = {}
+    for i, p in enumerate((4.0, 5.0, 6.0, 7.0)):
+        llr, info = F.generate_batch(code, batch, F.QAM16, p, snr_kind=F.ESN0_DB, seed=3, point=i)
         bits, its, _ = code.decode_batch(llr)
-        torch.cuda.synchronize()
-        errs = (bits.cpu().numpy() != info).sum(1)
-        fe = int((errs > 0).sum())
-        points[esn0] = {"fer": fe / batch, "frame_errors": fe, "bit_errors": int(errs.sum()),
-                        "mean_iterations": float(its.float().mean().item()), "llr": llr, "info": info}
-    # the point nearest FER 1e-3 on a log scale (0 errors counts as 1e-5)
-    esn0 = min(points, key=lambda p: abs(math.log10(max(points[p]["fer"], 1e-5)) + 3.0))
-    llr = points[esn0]["llr"]
-    for p in points:
-        if p != esn0:
-            points[p].pop("llr")
-    mean_it = points[esn0]["mean_iterations"]
+        grid[str(p)] = {"fer": frame_errors(bits, info, F) / batch, "mean_iterations":
+                        round(float(its.float().mean().item()), 3)}
 
-    stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
-        code.decode_batch(llr)
-    torch.cuda.synchronize()
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # the learned decoder, trained on the device around the operating point
+    params = F.initial_params(code, LEARNED_ITERATIONS)
     t0 = time.perf_counter()
-    start.record(stream)
-    for _ in range(args.steps):
-        code.decode_batch(llr)
-    stop.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_total = start.elapsed_time(stop)
-    ms_per_step = ms_total / args.steps
-    value = k_info * batch / (ms_per_step / 1e3) / 1e9
+    hist = F.learned_train(code, params, steps=args.train_steps, batch=512, lr=1e-2, modulation=F.QAM16,
+                           snr_kind=F.ESN0_DB, snr_lo=esn0 - 0.1, snr_hi=esn0 + 0.3, seed=11)
+    train_s = time.perf_counter() - t0
 
-    # roofline of the decode kernel: algorithmic bytes per codeword (channel LLRs in, K/8
-    # decisions out, and per iteration every edge message read and written in FP32)
-    edges = 316 * 384
-    alg_bytes_cw = e * 4 + math.ceil(k_info / 8) + mean_it * 2 * edges * 4
-    achieved = alg_bytes_cw * batch / (ms_per_step / 1e3) / 1e9
-    lds_bytes_cw = mean_it * edges * 16
+    # the timed batch and an independent evaluation set at the operating point
+    llr, info = F.generate_batch(code, batch, F.QAM16, esn0, snr_kind=F.ESN0_DB, seed=5, point=0)
+    ev = {"plain": [0, 0, 0.0], "learned": [0, 0, 0.0]}
+    for part in range(8):
+        l2, i2 = F.generate_batch(code, 4096, F.QAM16, esn0, snr_kind=F.ESN0_DB, seed=6, point=0, trial0=4096 * part)
+        b, its, _ = code.decode_batch(l2)
+        ev["plain"][0] += frame_errors(b, i2, F)
+        ev["plain"][1] += 4096
+        ev["plain"][2] += float(its.double().sum().item())
+        b, its, _ = F.learned_decode_batch(code, params, l2, fallback=True)
+        ev["learned"][0] += frame_errors(b, i2, F)
+        ev["learned"][1] += 4096
+        ev["learned"][2] += float(its.double().sum().item())
+
+    _, its_plain, _ = code.decode_batch(llr)
+    _, its_learned, _ = F.learned_decode_batch(code, params, llr, fallback=True)
+    it_plain = float(its_plain.float().mean().item())
+    it_learned = float(its_learned.float().mean().item())
+    ms_plain, wall_plain = timed(lambda: code.decode_batch(llr), args.steps, args.warmup, torch)
+    ms_learned, wall_learned = timed(lambda: F.learned_decode_batch(code, params, llr, fallback=True),
+                                     args.steps, args.warmup, torch)
+    gbps_plain = K_INFO * batch / (ms_plain / 1e3) / 1e9
+    gbps_learned = K_INFO * batch / (ms_learned / 1e3) / 1e9
+
+    # roofline of the layered kernel (one launch per plain step): algorithmic bytes per
+    # codeword = channel LLRs in + K/8 decisions out + per iteration every edge message
+    # read and written in FP32 (the HBM yardstick of a decoder that streams its messages)
+    edges = BG1_EDGES * Z
+    alg_bytes = batch * (E * 4 + math.ceil(K_INFO / 8) + it_plain * 2 * edges * 4)
+    achieved = alg_bytes / (ms_plain / 1e3) / 1e9
+    lds_bytes = batch * it_plain * edges * 8  # posterior read + write per edge and iteration
+    wil = lambda fe, n: [round(x, 7) for x in F.wilson(fe, n, 0.95)]
     out = {
         "metric": "decoded Gbit/s at FER 1e-3, learned vs plain min-sum, NR BG1 Z=384",
-        "value": round(value, 4),
+        "value": round(gbps_learned, 4),
         "unit": "Gbit/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(ms_learned, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: random info bits, NR encoder, Gray 16-QAM, AWGN, max-log LLRs (on device)",
+        "data": "synthetic: device Philox trial generator (random info bits, NR encoder + rate matching + "
+                "interleaver, Gray 16-QAM, AWGN, max-log LLRs)",
         "config": {
-            "workload": "plain layered NMS a=0.75, NR BG1 Z=384 K=8448 E=25344 rate 1/3, 16-QAM, 10 it",
-            "decoder": "plain", "global_batch": batch, "esn0_db": esn0, "mean_iterations": round(mean_it, 3),
-            "fer_by_esn0": {str(p): v["fer"] for p, v in points.items()},
-            "wall_s_timed": round(wall, 4),
+            "workload": "NR BG1 Z=384 K=8448 E=25344 rate 1/3, 16-QAM, layered min-sum, 4096 codewords/step, "
+                        "Es/N0 at plain FER 1e-3",
+            "global_batch": batch,
+            "esn0_db": esn0,
+            "plain": {"decoder": "normalized min-sum a=0.75, 10 it", "gbit_s": round(gbps_plain, 4),
+                      "ms_per_step": round(ms_plain, 4), "mean_iterations": round(it_plain, 3),
+                      "fer": ev["plain"][0] / ev["plain"][1], "fer_wilson95": wil(*ev["plain"][:2]),
+                      "frames": ev["plain"][1]},
+            "learned": {"decoder": f"offset/scale min-sum, {LEARNED_ITERATIONS} it, plain fallback",
+                        "gbit_s": round(gbps_learned, 4), "ms_per_step": round(ms_learned, 4),
+                        "mean_iterations": round(it_learned, 3),
+                        "fer": ev["learned"][0] / ev["learned"][1], "fer_wilson95": wil(*ev["learned"][:2]),
+                        "frames": ev["learned"][1], "train_steps": args.train_steps,
+                        "train_s": round(train_s, 2), "loss_first_last": [round(float(hist[:20].mean()), 5),
+                                                                          round(float(hist[-20:].mean()), 5)]},
+            "speedup_learned_vs_plain": round(gbps_learned / gbps_plain, 4),
+            "iteration_ratio_learned_vs_plain": round(it_learned / it_plain, 4),
+            "plain_fer_calibration": {str(p): [fe, n] for p, (fe, n) in calib.items()},
+            "plain_fer_by_esn0_grid": grid,
+            "wall_s_timed": round(wall_plain + wall_learned, 4),
         },
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "lds": {"achieved": round(lds_bytes_cw * batch / (ms_per_step / 1e3) / 1e9, 2),
-                             "peak": LDS_PEAK_GBS, "unit": "GB/s"}},
+                     "kernel": "decode_layered_kernel<1> (plain step)",
+                     "lds": {"achieved": round(lds_bytes / (ms_plain / 1e3) / 1e9, 2), "peak": LDS_PEAK_GBS,
+                             "unit": "GB/s"}},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline((k_info, e, 1, 0, 4), llr.cpu().numpy(), args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(llr.cpu().numpy(), args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

@@ -94,6 +94,7 @@ SIGNATURES += [
                                        C.POINTER(C.c_double)]),
     ("cf_results_metric", C.c_int, [C.c_void_p, C.c_size_t, C.c_char_p, C.c_double, C.POINTER(Metric)]),
     ("cf_results_free", None, [C.c_void_p]),
+    ("cf_wilson", C.c_int, [C.c_uint64, C.c_uint64, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
 ]
 
 BPSK, QPSK, QAM16, QAM64 = 1, 2, 4, 6
@@ -451,6 +452,13 @@ class Runner:
         _check(lib().cf_nr_campaign(code._h, C.byref(link), C.byref(cfg), stop, n_stop, axes, len(grid.parts),
                                     C.byref(res)))
         return _collect(res, _conditions(grid), self._conf, metrics)
+
+
+def wilson(errors, total, confidence=0.95):
+    """ErrorRate::wilson (crates/com-sim/src/stats.rs:74-87)."""
+    lo, hi = C.c_double(), C.c_double()
+    _check(lib().cf_wilson(int(errors), int(total), confidence, C.byref(lo), C.byref(hi)))
+    return lo.value, hi.value
 
 
 def generate_batch(code, count, modulation, snr_db, snr_kind=EBN0_DB, seed=1, point=0, trial0=0, stream=None):

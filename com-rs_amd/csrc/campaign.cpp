@@ -365,4 +365,16 @@ cf_status cf_results_metric(const cf_results* res, size_t i, const char* name, d
 
 void cf_results_free(cf_results* res) { delete res; }
 
+cf_status cf_wilson(uint64_t errors, uint64_t total, double confidence, double* lo, double* hi) {
+    return guarded([&] {
+        if (!lo || !hi) fail(Kind::Config, "null output");
+        if (errors > total) fail(Kind::Config, "more errors than trials");
+        sim::check_confidence(confidence, "the interval");
+        sim::ErrorRate r{errors, total};
+        auto [l, h] = r.wilson(confidence);
+        *lo = l;
+        *hi = h;
+    });
+}
+
 }  // extern "C"

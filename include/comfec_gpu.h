@@ -211,6 +211,10 @@ size_t cf_results_count(const cf_results* res);
 /* status: 0 completed, 1 stopped, 2 skipped, 3 dev run (runner.rs:58-68) */
 cf_status cf_results_condition(const cf_results* res, size_t i, int* status, uint64_t* trials,
                                double* wall_s);
+/* ErrorRate::wilson (crates/com-sim/src/stats.rs:74-87): the Wilson score interval of
+   `errors` among `total` at `confidence`, z from Cody's erfc (special.rs:180-293). */
+cf_status cf_wilson(uint64_t errors, uint64_t total, double confidence, double* lo, double* hi);
+
 cf_status cf_results_metric(const cf_results* res, size_t i, const char* name, double confidence,
                             cf_metric* out);
 void cf_results_free(cf_results* res);

@@ -148,3 +148,11 @@ def test_a_failing_trial_stops_the_run_with_context():
     with pytest.raises(F.ComfecError) as e:
         F.Runner().batch(4).stop(F.Stop.max_trials(8)).run("bad", scen, F.Grid().axis("x", [3]))
     assert e.value.kind == "Numerical" and "x=3" in str(e.value) and "no convergence" in str(e.value)
+
+
+def test_wilson_of_the_library_equals_the_oracle():
+    for k, n in [(0, 1000), (37, 1000), (1, 1), (200, 1_000_000), (999, 1000)]:
+        for c in (0.95, 0.99):
+            assert F.wilson(k, n, c) == pytest.approx(O.wilson(k, n, c), rel=1e-15, abs=0)
+    with pytest.raises(F.ComfecError):
+        F.wilson(5, 3)
