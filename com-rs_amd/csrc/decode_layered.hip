@@ -94,7 +94,19 @@ __device__ inline float with_sign(float mag, uint32_t bit) {
     return __uint_as_float(__float_as_uint(mag) | (bit << 31));
 }
 
-// ldpc.rs:522-531 for the row of lane r in layer I
+// ldpc.rs:522-531 for the row of lane r in layer I.
+//
+// Exactness of the shortcuts, given that no posterior or message is ever -0.0 (the LLRs
+// are summed onto +0.0 when they are placed, and a difference or sum of two values that are
+// not -0.0 is not -0.0 except for -0.0 - +0.0):
+//  - the sign bit of a message is the reference's test `l < 0.0`, so the output sign of
+//    edge q, the product of the other input signs, is bit 31 of (input_q XOR all inputs);
+//  - the edge that takes the second minimum is told by |input_q| == m1: it is the first
+//    minimum, or a tie with it, and a tie makes m2 = m1, so either way the magnitude is the
+//    reference's g(q == i1 ? m2 : m1);
+//  - m2 = med3(m1, m2, |input|) is the strict-< update of the scan of ldpc.rs:471-480.
+// The stored state keeps the output sign bits (edge q at bit deg-1-q, shifted in with a
+// funnel shift) and one index whose magnitude is g(m2).
 template <int BG, int I>
 __device__ __forceinline__ void layer(float* __restrict__ post, int r, int z, const uint32_t* __restrict__ shift,
                                       Messages<Graph<BG>::rows>& msg, float a, float b) {
@@ -102,38 +114,43 @@ __device__ __forceinline__ void layer(float* __restrict__ post, int r, int z, co
     constexpr int deg = row_begin<BG>(I + 1) - e0;
     const float g1o = msg.g1[I], g2o = msg.g2[I];
     const uint32_t pko = msg.pk[I];
-    const int i1o = int(pko >> 24);
+    const uint32_t i1o = pko >> 24;
     float v2c[deg];
     int addr[deg];
     float m1 = INFINITY, m2 = INFINITY;
-    int i1 = 0;
-    uint32_t neg = 0;
+    uint32_t parity = 0;
 #pragma unroll
     for (int q = 0; q < deg; ++q) {
-        addr[q] = int(Graph<BG>::t[e0 + q].j) * z + wrap_add(r, int(shift[e0 + q]), z);
-        const float old = with_sign(q == i1o ? g2o : g1o, (pko >> q) & 1u);
+        // (r + shift) mod z: the unsigned min of t and t - z (which wraps when t < z)
+        const uint32_t t = uint32_t(r) + shift[e0 + q];
+        const uint32_t rr = min(t, t - uint32_t(z));
+        addr[q] = int(Graph<BG>::t[e0 + q].j) * z + int(rr);
+        const float mag = uint32_t(q) == i1o ? g2o : g1o;
+        const float old = __uint_as_float(__float_as_uint(mag) ^ ((pko << (32 - deg + q)) & 0x80000000u));
         const float in = post[addr[q]] - old;
         v2c[q] = in;
+        parity ^= __float_as_uint(in);
         const float m = fabsf(in);
-        neg |= uint32_t(in < 0.0f) << q;
-        // strict <: the first minimum wins; a tie leaves m2 = m1
-        const bool lower = m < m1;
-        m2 = lower ? m1 : fminf(m2, m);
-        i1 = lower ? q : i1;
-        m1 = lower ? m : m1;
+        m2 = __builtin_amdgcn_fmed3f(m1, m2, m);
+        m1 = fminf(m1, m);
     }
-    const uint32_t total = __popc(neg) & 1u;
-    const uint32_t out_signs = total ? (~neg & ((1u << deg) - 1u)) : neg;
     const float g1 = a * fmaxf(m1 - b, 0.0f);
     const float g2 = a * fmaxf(m2 - b, 0.0f);
+    parity &= 0x80000000u;
+    uint32_t signs = 0, i1 = 0;
 #pragma unroll
     for (int q = 0; q < deg; ++q) {
-        const float out = with_sign(q == i1 ? g2 : g1, (out_signs >> q) & 1u);
+        const bool first = fabsf(v2c[q]) == m1;
+        const float mag = first ? g2 : g1;
+        i1 = first ? uint32_t(q) : i1;
+        const uint32_t sb = __float_as_uint(v2c[q]) ^ parity;
+        signs = __builtin_amdgcn_alignbit(signs, sb, 31u);
+        const float out = __uint_as_float(__float_as_uint(mag) ^ (sb & 0x80000000u));
         post[addr[q]] = v2c[q] + out;
     }
     msg.g1[I] = g1;
     msg.g2[I] = g2;
-    msg.pk[I] = out_signs | (uint32_t(i1) << 24);
+    msg.pk[I] = signs | (i1 << 24);
 }
 
 template <int BG, int I>
