@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libcomfec_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["nr_code.cpp", "capi.cpp", "sim.cpp", "campaign.cpp", "learned_api.cpp", "decode_generic.hip", "decode_layered.hip", "trials.hip", "learned.hip"]
+SOURCES = ["nr_code.cpp", "capi.cpp", "sim.cpp", "campaign.cpp", "learned_api.cpp", "batch.cpp", "decode_generic.hip", "decode_layered.hip", "decode_layered_f32.hip", "decode_layered_f16.hip", "trials.hip", "learned.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -68,7 +68,8 @@ class RunnerConfig(C.Structure):
 
 
 class NrLink(C.Structure):
-    _fields_ = [("modulation", C.c_int), ("snr_kind", C.c_int), ("d_learned_params", C.c_void_p)]
+    _fields_ = [("modulation", C.c_int), ("snr_kind", C.c_int), ("d_learned_params", C.c_void_p),
+                ("learned_iterations", C.c_uint32), ("fallback", C.c_int)]
 
 
 class Metric(C.Structure):
@@ -233,15 +234,20 @@ class NrLdpc:
                                  out.size, C.byref(it)))
         return out, it.value
 
-    def decode_batch(self, llr, *, want_post=False, want_bits=True, stream=None):
+    def decode_batch(self, llr, *, want_post=False, want_bits=True, half=False, stream=None):
         """Decode a (count, E) torch tensor on the GPU (float32 or float64 messages, the type of
-        llr): returns (bits (count, K') uint8, iterations (count,) int32, post (count, N) or None)."""
+        llr; half=True: float32 LLRs with the messages stored in binary16): returns
+        (bits (count, K') uint8, iterations (count,) int32, post (count, N) or None)."""
         import torch
         if llr.dim() != 2 or llr.shape[1] != self.dims.e:
             raise ComfecError(CF_SHAPE, f"inconsistent shapes: {llr.shape[-1]} LLRs for a block of {self.dims.e}")
         if not llr.is_cuda or not llr.is_contiguous():
             raise ComfecError(CF_CONFIG, "invalid configuration: the LLRs must be a contiguous device tensor")
         msg = {torch.float64: MSG_F64, torch.float32: MSG_F32}[llr.dtype]
+        if half:
+            if llr.dtype != torch.float32:
+                raise ComfecError(CF_CONFIG, "invalid configuration: FP16 storage takes float32 LLRs")
+            msg = MSG_F16
         count = llr.shape[0]
         dev = llr.device
         bits = torch.empty((count, self.dims.k_info), dtype=torch.uint8, device=dev) if want_bits else None
@@ -440,11 +446,13 @@ class Runner:
         _check(status)
         return _collect(res, conds, self._conf, metrics)
 
-    def run_nr(self, code, grid, modulation=BPSK, snr_kind=EBN0_DB, learned_params=None,
+    def run_nr(self, code, grid, modulation=BPSK, snr_kind=EBN0_DB, learned_params=None, fallback=False,
                metrics=("ber", "fer", "iterations")):
-        """Runner::run with the NR link on the GPU as the scenario (cf_nr_campaign)."""
+        """Runner::run with the NR link on the GPU as the scenario (cf_nr_campaign). With
+        learned_params (iterations, rows, 2), the learned decoder runs its iterations and, with
+        fallback, the code's own min-sum on the codewords whose syndrome still fails."""
         link = NrLink(modulation, snr_kind, C.c_void_p(learned_params.data_ptr()) if learned_params is not None
-                      else None)
+                      else None, learned_params.shape[0] if learned_params is not None else 0, int(fallback))
         axes, keep = grid._axes()
         stop, n_stop = self._stop_arr()
         cfg = self._config()
@@ -551,4 +559,103 @@ def learned_decode_batch(code, params, llr, fallback=True, stream=None):
     ok = torch.empty(count, dtype=torch.uint8, device="cuda")
     _check(lib().cf_learned_decode_batch(code._h, params.shape[0], _ptr(params), int(fallback), _ptr(llr), count,
                                          _ptr(bits), _ptr(its), _ptr(ok), _stream_ptr(stream)))
+    return bits, its, ok
+
+
+# ---- batches of many codes ----------------------------------------------------------------------
+
+
+class PlanDims(C.Structure):
+    _fields_ = [("llr_total", C.c_uint64), ("bits_total", C.c_uint64), ("codewords", C.c_uint32),
+                ("classes", C.c_uint32), ("tasks", C.c_uint32), ("threads", C.c_uint32), ("grid", C.c_uint32),
+                ("lds_bytes", C.c_uint32), ("msg_type", C.c_int)]
+
+
+SIGNATURES += [
+    ("cf_batch_plan_create", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
+                                       C.POINTER(C.c_void_p)]),
+    ("cf_batch_plan_destroy", None, [C.c_void_p]),
+    ("cf_batch_plan_dims_get", C.c_int, [C.c_void_p, C.POINTER(PlanDims)]),
+    ("cf_batch_plan_set_threads", C.c_int, [C.c_void_p, C.c_uint32]),
+    ("cf_batch_plan_tune", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                     C.POINTER(C.c_double), C.c_void_p]),
+    ("cf_batch_plan_run", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                    C.c_void_p]),
+    ("cf_decode_batch_uneven", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+]
+
+
+def _plan_inputs(codes, llr_offsets, max_iterations):
+    handles = (C.c_void_p * len(codes))(*[c._h for c in codes])
+    offs = np.ascontiguousarray(llr_offsets, np.uint64)
+    if offs.shape != (len(codes) + 1,):
+        raise ComfecError(CF_SHAPE, f"inconsistent shapes: {offs.size} offsets for {len(codes)} codewords")
+    lim = None if max_iterations is None else np.ascontiguousarray(max_iterations, np.uint32)
+    return handles, offs, lim
+
+
+class BatchPlan:
+    """A decode plan for codewords of many codes (cf_batch_plan_*): codes[i] decodes
+    llr[llr_offsets[i]:llr_offsets[i+1]]; the decisions are concatenated in codeword order."""
+
+    def __init__(self, codes, llr_offsets, max_iterations=None, half=False):
+        self.codes = list(codes)  # the handles outlive the plan
+        handles, offs, lim = _plan_inputs(self.codes, llr_offsets, max_iterations)
+        self._h = C.c_void_p()
+        _check(lib().cf_batch_plan_create(handles, offs.ctypes.data_as(C.c_void_p),
+                                          lim.ctypes.data_as(C.c_void_p) if lim is not None else None,
+                                          len(self.codes), MSG_F16 if half else MSG_F32, C.byref(self._h)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.cf_batch_plan_destroy(h)
+            self._h = None
+
+    @property
+    def dims(self):
+        d = PlanDims()
+        _check(lib().cf_batch_plan_dims_get(self._h, C.byref(d)))
+        return d
+
+    def outputs(self):
+        import torch
+        d = self.dims
+        return (torch.empty(d.bits_total, dtype=torch.uint8, device="cuda"),
+                torch.empty(d.codewords, dtype=torch.int32, device="cuda"),
+                torch.empty(d.codewords, dtype=torch.uint8, device="cuda"))
+
+    def set_threads(self, threads):
+        _check(lib().cf_batch_plan_set_threads(self._h, threads))
+        return self
+
+    def tune(self, llr, out=None, repeats=3, stream=None):
+        """Autotune the workgroup width on this input; returns (best ms, outputs)."""
+        bits, its, ok = out if out is not None else self.outputs()
+        ms = C.c_double()
+        _check(lib().cf_batch_plan_tune(self._h, _ptr(llr), _ptr(bits), _ptr(its), _ptr(ok), repeats, C.byref(ms),
+                                        _stream_ptr(stream)))
+        return ms.value, (bits, its, ok)
+
+    def run(self, llr, out=None, graph=False, stream=None):
+        """Decode; returns (bits (sum K',), iterations (count,), ok (count,))."""
+        bits, its, ok = out if out is not None else self.outputs()
+        _check(lib().cf_batch_plan_run(self._h, _ptr(llr), _ptr(bits), _ptr(its), _ptr(ok), int(graph),
+                                       _stream_ptr(stream)))
+        return bits, its, ok
+
+
+def decode_batch_uneven(codes, llr, llr_offsets, max_iterations=None, half=False, stream=None):
+    """One-shot cf_decode_batch_uneven: (bits (sum K',), iterations, ok)."""
+    import torch
+    handles, offs, lim = _plan_inputs(codes, llr_offsets, max_iterations)
+    total = sum(c.dims.k_info for c in codes)
+    bits = torch.empty(total, dtype=torch.uint8, device="cuda")
+    its = torch.empty(len(codes), dtype=torch.int32, device="cuda")
+    ok = torch.empty(len(codes), dtype=torch.uint8, device="cuda")
+    _check(lib().cf_decode_batch_uneven(handles, offs.ctypes.data_as(C.c_void_p),
+                                        lim.ctypes.data_as(C.c_void_p) if lim is not None else None, len(codes),
+                                        MSG_F16 if half else MSG_F32, _ptr(llr), _ptr(bits), _ptr(its), _ptr(ok),
+                                        _stream_ptr(stream)))
     return bits, its, ok

@@ -1,0 +1,412 @@
+// Launches of the layered min-sum kernel: batches of one code (cf_decode_batch, the learned
+// decoder, the campaigns) and plans for batches of many codes (cf_batch_plan_*): the task
+// list, its device copy, the autotuned workgroup width and the captured HIP graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "capi_internal.hpp"
+#include "kernels.hpp"
+#include "nr_code.hpp"
+
+using namespace comfec;
+using namespace comfec::detail;
+
+namespace comfec {
+namespace detail {
+
+// The class of a code: its tables on the device and its min-sum rule.
+BatchClass batch_class(const NrCode& c, const DecoderSettings& s, int device, int g) {
+    const NrCode::DeviceTables& t = c.device_tables(device);
+    BatchClass k{};
+    k.bg = c.bg;
+    k.z = int32_t(c.z);
+    k.n = int32_t(c.n);
+    k.k = int32_t(c.k);
+    k.k_info = int32_t(c.k_info);
+    k.e = int32_t(c.e);
+    k.g = g;
+    k.max_iterations = int32_t(s.max_iterations);
+    // g(m) = a max(m - b, 0): (alpha, 0) normalized, (1, beta) offset min-sum
+    k.a = s.rule == 1 ? float(s.alpha) : 1.0f;
+    k.b = s.rule == 1 ? 0.0f : float(s.alpha);
+    k.shift32 = t.shift32;
+    k.col = t.col;
+    k.row_start = t.row_start;
+    k.src_start = t.src_start;
+    k.src = t.src;
+    return k;
+}
+
+namespace {
+
+int round64(int x) { return (x + 63) / 64 * 64; }
+
+// codewords per task for lifting size z in a workgroup of `threads` lanes
+int group_of(int z, int threads) { return std::max(1, std::min(threads / z, kMaxGroup)); }
+
+}  // namespace
+
+// The layered kernel on `count` codewords of one code: a workgroup of the next multiple of
+// 64 lanes above Z takes floor(lanes / Z) codewords at a time.
+void run_layered(const cf_code* h, int device, const LayeredRun& r, hipStream_t stream) {
+    const NrCode& c = h->code;
+    if (!layered_supported(c.bg, int(c.z)))
+        fail(Kind::Config, "the layered kernel needs a lifting size of at most " + std::to_string(kMaxThreads));
+    if (r.count <= 0) return;
+    const int threads = std::min(round64(int(c.z)), kMaxThreads);
+    const int g = group_of(int(c.z), threads);
+    LayeredArgs a{};
+    a.one = batch_class(c, h->settings(), device, g);
+    a.n_tasks = int32_t((r.count + g - 1) / g);
+    a.count = r.count;
+    a.params = r.params;
+    a.learned_iterations = int32_t(r.learned_iterations);
+    a.fallback = r.fallback ? 1 : 0;
+    a.llr = r.llr;
+    a.bits = r.bits;
+    a.post_out = r.post;
+    a.iterations = r.iterations;
+    a.ok = r.ok;
+    LayeredLaunch l{c.bg, r.half, g == 1, c.z == 384 && g == 1 ? 384 : 0, threads, int(a.n_tasks),
+                    layered_lds_bytes(c.bg, int(c.z), g, r.half)};
+    check_hip(launch_layered(a, l, stream, true), "layered decode launch");
+}
+
+}  // namespace detail
+}  // namespace comfec
+
+// ---- plans for batches of many codes ---------------------------------------------------
+
+struct cf_batch_plan {
+    std::mutex mu;
+    int device = 0;
+    bool half = false;
+    size_t count = 0;
+    uint64_t llr_total = 0, bits_total = 0;
+    std::vector<const cf_code*> class_code;  // per class
+    std::vector<DecoderSettings> class_set;
+    std::vector<uint32_t> cls_of;            // per codeword
+    std::vector<uint16_t> max_it;
+    std::vector<uint64_t> llr_off, bit_off;
+    int bg_mask = 0, max_z = 0;
+    // the configuration
+    int threads = 0;
+    std::vector<BatchClass> classes;
+    std::vector<BatchTask> tasks;
+    std::vector<uint32_t> order;
+    LayeredLaunch launch{};
+    // device copies: classes, tasks, order, offsets, limits and the task counter
+    void* dev = nullptr;
+    size_t dev_bytes = 0;
+    LayeredArgs args{};
+    // the captured graph and the buffers it was captured with
+    hipStream_t capture_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void* g_llr = nullptr;
+    const void* g_bits = nullptr;
+    const void* g_its = nullptr;
+    const void* g_ok = nullptr;
+
+    ~cf_batch_plan() {
+        drop_graph();
+        if (capture_stream) (void)hipStreamDestroy(capture_stream);
+        if (dev) (void)hipFree(dev);
+    }
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+};
+
+namespace {
+
+size_t up256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Tasks for a workgroup width: every code's codewords in groups of G, those with the larger
+// iteration limits together, and the tasks in decreasing order of work (the largest first,
+// so that the tail of the launch is short).
+void configure(cf_batch_plan& p, int threads) {
+    if (threads < round64(p.max_z) || threads > kMaxThreads || threads % 64 != 0)
+        fail(Kind::Config, "the workgroup width must be a multiple of 64 from " + std::to_string(round64(p.max_z)) +
+                               " to " + std::to_string(kMaxThreads) + ", got " + std::to_string(threads));
+    const size_t nc = p.class_code.size();
+    p.classes.assign(nc, BatchClass{});
+    size_t lds = 0;
+    bool solo = true, all384 = true;
+    for (size_t k = 0; k < nc; ++k) {
+        const NrCode& c = p.class_code[k]->code;
+        const int g = std::max(1, std::min(threads / int(c.z), kMaxGroup));
+        p.classes[k] = batch_class(c, p.class_set[k], p.device, g);
+        lds = std::max(lds, layered_lds_bytes(c.bg, int(c.z), g, p.half));
+        solo = solo && g == 1;
+        all384 = all384 && c.z == 384;
+    }
+    const int zc = solo && all384 && p.bg_mask != 3 ? 384 : 0;
+    std::vector<std::vector<uint32_t>> members(nc);
+    for (size_t i = 0; i < p.count; ++i) members[p.cls_of[i]].push_back(uint32_t(i));
+    struct Pending {
+        BatchTask t;
+        double cost;
+        std::vector<uint32_t> ids;
+    };
+    std::vector<Pending> pend;
+    for (size_t k = 0; k < nc; ++k) {
+        auto& m = members[k];
+        std::stable_sort(m.begin(), m.end(), [&](uint32_t x, uint32_t y) { return p.max_it[x] > p.max_it[y]; });
+        const int g = p.classes[k].g;
+        const NrCode& c = p.class_code[k]->code;
+        for (size_t s = 0; s < m.size(); s += size_t(g)) {
+            Pending q;
+            q.t = BatchTask{uint32_t(k), 0, uint32_t(std::min(m.size() - s, size_t(g))), 0};
+            q.ids.assign(m.begin() + long(s), m.begin() + long(s + q.t.count));
+            // a workgroup runs until its slowest codeword stops: limit x edges x Z per codeword
+            q.cost = double(std::max<uint16_t>(p.max_it[q.ids[0]], 1)) * double(c.edges_b) * double(c.z);
+            pend.push_back(std::move(q));
+        }
+    }
+    std::stable_sort(pend.begin(), pend.end(), [](const Pending& x, const Pending& y) { return x.cost > y.cost; });
+    p.tasks.clear();
+    p.order.clear();
+    for (auto& q : pend) {
+        q.t.first = uint32_t(p.order.size());
+        p.tasks.push_back(q.t);
+        p.order.insert(p.order.end(), q.ids.begin(), q.ids.end());
+    }
+    int per_cu = 0, cus = 0;
+    check_hip(layered_occupancy(p.bg_mask, p.half, solo, zc, threads, lds, &per_cu), "occupancy");
+    check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p.device), "device attribute");
+    per_cu = std::max(per_cu, 1);
+    const int grid = int(std::min<size_t>(p.tasks.size(), size_t(per_cu) * size_t(std::max(cus, 1))));
+    p.threads = threads;
+    p.launch = LayeredLaunch{p.bg_mask, p.half, solo, zc, threads, std::max(grid, 1), lds};
+
+    // one device buffer: [queue | classes | tasks | order | llr_off | bit_off | max_it]
+    const size_t o_cls = 256, o_task = o_cls + up256(nc * sizeof(BatchClass));
+    const size_t o_order = o_task + up256(p.tasks.size() * sizeof(BatchTask));
+    const size_t o_llr = o_order + up256(p.count * 4), o_bit = o_llr + up256(p.count * 8);
+    const size_t o_it = o_bit + up256(p.count * 8), total = o_it + up256(p.count * 2);
+    check_hip(hipDeviceSynchronize(), "synchronize");  // no launch still reads the old tables
+    p.drop_graph();
+    if (total > p.dev_bytes) {
+        if (p.dev) (void)hipFree(p.dev);
+        p.dev = nullptr;
+        p.dev_bytes = 0;
+        check_hip(hipMalloc(&p.dev, total), "hipMalloc");
+        p.dev_bytes = total;
+    }
+    std::vector<uint8_t> host(total, 0);
+    std::memcpy(host.data() + o_cls, p.classes.data(), nc * sizeof(BatchClass));
+    std::memcpy(host.data() + o_task, p.tasks.data(), p.tasks.size() * sizeof(BatchTask));
+    std::memcpy(host.data() + o_order, p.order.data(), p.count * 4);
+    std::memcpy(host.data() + o_llr, p.llr_off.data(), p.count * 8);
+    std::memcpy(host.data() + o_bit, p.bit_off.data(), p.count * 8);
+    std::memcpy(host.data() + o_it, p.max_it.data(), p.count * 2);
+    check_hip(hipMemcpy(p.dev, host.data(), total, hipMemcpyHostToDevice), "copy plan");
+    uint8_t* d = static_cast<uint8_t*>(p.dev);
+    LayeredArgs& a = p.args;
+    a = LayeredArgs{};
+    a.classes = reinterpret_cast<const BatchClass*>(d + o_cls);
+    a.tasks = reinterpret_cast<const BatchTask*>(d + o_task);
+    a.n_tasks = int32_t(p.tasks.size());
+    a.queue = reinterpret_cast<uint32_t*>(d);
+    a.order = reinterpret_cast<const uint32_t*>(d + o_order);
+    a.llr_off = reinterpret_cast<const uint64_t*>(d + o_llr);
+    a.bit_off = reinterpret_cast<const uint64_t*>(d + o_bit);
+    a.max_it = reinterpret_cast<const uint16_t*>(d + o_it);
+}
+
+// the counter fill and the decode launch
+void enqueue(cf_batch_plan& p, const float* llr, uint8_t* bits, uint32_t* its, uint8_t* ok, hipStream_t s,
+             bool set_attribute) {
+    LayeredArgs a = p.args;
+    a.llr = llr;
+    a.bits = bits;
+    a.iterations = its;
+    a.ok = ok;
+    check_hip(launch_zero(reinterpret_cast<uint64_t*>(a.queue), 1, s), "zero launch");
+    check_hip(launch_layered(a, p.launch, s, set_attribute), "batch decode launch");
+}
+
+void run_plan(cf_batch_plan& p, const float* llr, uint8_t* bits, uint32_t* its, uint8_t* ok, bool use_graph,
+              hipStream_t s) {
+    if (p.count == 0) return;
+    if (!llr) fail(Kind::Config, "null LLR buffer");
+    if (!use_graph) {
+        enqueue(p, llr, bits, its, ok, s, true);
+        return;
+    }
+    if (!p.exec || p.g_llr != llr || p.g_bits != bits || p.g_its != its || p.g_ok != ok) {
+        p.drop_graph();
+        if (!p.capture_stream) check_hip(hipStreamCreate(&p.capture_stream), "hipStreamCreate");
+        // kernel attributes are set outside the capture
+        check_hip(layered_prepare(p.launch), "kernel attribute");
+        check_hip(hipStreamBeginCapture(p.capture_stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        try {
+            enqueue(p, llr, bits, its, ok, p.capture_stream, false);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(p.capture_stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        check_hip(hipStreamEndCapture(p.capture_stream, &p.graph), "end capture");
+        check_hip(hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0), "graph instantiate");
+        p.g_llr = llr;
+        p.g_bits = bits;
+        p.g_its = its;
+        p.g_ok = ok;
+    }
+    check_hip(hipGraphLaunch(p.exec, s), "graph launch");
+}
+
+cf_batch_plan* create_plan(const cf_code* const* codes, const uint64_t* llr_offsets, const uint32_t* max_iterations,
+                           size_t count, int msg_type) {
+    if (!codes || !llr_offsets) fail(Kind::Config, "null code list or offsets");
+    if (msg_type != CF_MSG_F32 && msg_type != CF_MSG_F16)
+        fail(Kind::Config, "a batch plan decodes with CF_MSG_F32 or CF_MSG_F16 messages, got " +
+                               std::to_string(msg_type));
+    auto p = std::make_unique<cf_batch_plan>();
+    p->device = current_device();
+    p->half = msg_type == CF_MSG_F16;
+    p->count = count;
+    p->cls_of.resize(count);
+    p->max_it.resize(count);
+    p->llr_off.resize(count);
+    p->bit_off.resize(count);
+    if (llr_offsets[0] != 0) fail(Kind::Config, "the LLR offsets start at 0");
+    for (size_t i = 0; i < count; ++i) {
+        const cf_code* h = codes[i];
+        if (!h) fail(Kind::Config, "null code of codeword " + std::to_string(i));
+        auto it = std::find(p->class_code.begin(), p->class_code.end(), h);
+        if (it == p->class_code.end()) {
+            const DecoderSettings s = h->settings();
+            if (!s.layered || (s.rule != 1 && s.rule != 2) || !layered_supported(h->code.bg, int(h->code.z)))
+                fail(Kind::Config, "a batch plan runs the layered normalized or offset min-sum on lifting sizes "
+                                   "up to " + std::to_string(kMaxThreads));
+            p->class_code.push_back(h);
+            p->class_set.push_back(s);
+            p->bg_mask |= h->code.bg;
+            p->max_z = std::max(p->max_z, int(h->code.z));
+            it = p->class_code.end() - 1;
+        }
+        const size_t k = size_t(it - p->class_code.begin());
+        const NrCode& c = h->code;
+        if (llr_offsets[i + 1] < llr_offsets[i]) fail(Kind::Config, "the LLR offsets must not decrease");
+        expect_len(size_t(llr_offsets[i + 1] - llr_offsets[i]), c.e, "LLRs");
+        const uint32_t lim = max_iterations ? max_iterations[i] : p->class_set[k].max_iterations;
+        if (lim > 65535) fail(Kind::Config, "an iteration limit of at most 65535, got " + std::to_string(lim));
+        p->cls_of[i] = uint32_t(k);
+        p->max_it[i] = uint16_t(lim);
+        p->llr_off[i] = llr_offsets[i];
+        p->bit_off[i] = p->bits_total;
+        p->bits_total += c.k_info;
+    }
+    p->llr_total = count ? llr_offsets[count] : 0;
+    if (count) configure(*p, std::min(kMaxThreads, round64(p->max_z)));
+    return p.release();
+}
+
+}  // namespace
+
+extern "C" {
+
+cf_status cf_batch_plan_create(const cf_code* const* codes, const uint64_t* llr_offsets,
+                               const uint32_t* max_iterations, size_t count, int msg_type, cf_batch_plan** out) {
+    return guarded([&] {
+        if (!out) fail(Kind::Config, "null output handle");
+        *out = create_plan(codes, llr_offsets, max_iterations, count, msg_type);
+    });
+}
+
+void cf_batch_plan_destroy(cf_batch_plan* plan) { delete plan; }
+
+cf_status cf_batch_plan_dims_get(const cf_batch_plan* plan, cf_batch_plan_dims* out) {
+    return guarded([&] {
+        if (!plan || !out) fail(Kind::Config, "null plan");
+        *out = cf_batch_plan_dims{plan->llr_total,
+                                  plan->bits_total,
+                                  uint32_t(plan->count),
+                                  uint32_t(plan->class_code.size()),
+                                  uint32_t(plan->tasks.size()),
+                                  uint32_t(plan->threads),
+                                  uint32_t(plan->launch.grid),
+                                  uint32_t(plan->launch.lds),
+                                  plan->half ? CF_MSG_F16 : CF_MSG_F32};
+    });
+}
+
+cf_status cf_batch_plan_set_threads(cf_batch_plan* plan, uint32_t threads) {
+    return guarded([&] {
+        if (!plan) fail(Kind::Config, "null plan");
+        std::lock_guard<std::mutex> l(plan->mu);
+        if (plan->count) configure(*plan, int(threads));
+    });
+}
+
+cf_status cf_batch_plan_tune(cf_batch_plan* plan, const float* d_llr, uint8_t* d_bits, uint32_t* d_iterations,
+                             uint8_t* d_ok, uint32_t repeats, double* best_ms, void* stream) {
+    return guarded([&] {
+        if (!plan) fail(Kind::Config, "null plan");
+        std::lock_guard<std::mutex> l(plan->mu);
+        if (plan->count == 0) return;
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        repeats = std::max<uint32_t>(repeats, 1);
+        hipEvent_t e0, e1;
+        check_hip(hipEventCreate(&e0), "event");
+        check_hip(hipEventCreate(&e1), "event");
+        std::unique_ptr<void, void (*)(void*)> g0(e0, [](void* e) { (void)hipEventDestroy(hipEvent_t(e)); });
+        std::unique_ptr<void, void (*)(void*)> g1(e1, [](void* e) { (void)hipEventDestroy(hipEvent_t(e)); });
+        int best = 0;
+        float best_t = 0.0f;
+        for (int t = round64(plan->max_z); t <= kMaxThreads; t += 64) {
+            configure(*plan, t);
+            run_plan(*plan, d_llr, d_bits, d_iterations, d_ok, false, s);  // warm-up
+            check_hip(hipEventRecord(e0, s), "event");
+            for (uint32_t i = 0; i < repeats; ++i) run_plan(*plan, d_llr, d_bits, d_iterations, d_ok, false, s);
+            check_hip(hipEventRecord(e1, s), "event");
+            check_hip(hipEventSynchronize(e1), "event");
+            float ms = 0.0f;
+            check_hip(hipEventElapsedTime(&ms, e0, e1), "event");
+            ms /= float(repeats);
+            if (best == 0 || ms < best_t) {
+                best = t;
+                best_t = ms;
+            }
+        }
+        configure(*plan, best);
+        if (best_ms) *best_ms = double(best_t);
+    });
+}
+
+cf_status cf_batch_plan_run(cf_batch_plan* plan, const float* d_llr, uint8_t* d_bits, uint32_t* d_iterations,
+                            uint8_t* d_ok, int use_graph, void* stream) {
+    return guarded([&] {
+        if (!plan) fail(Kind::Config, "null plan");
+        std::lock_guard<std::mutex> l(plan->mu);
+        run_plan(*plan, d_llr, d_bits, d_iterations, d_ok, use_graph != 0, static_cast<hipStream_t>(stream));
+    });
+}
+
+cf_status cf_decode_batch_uneven(const cf_code* const* codes, const uint64_t* llr_offsets,
+                                 const uint32_t* max_iterations, size_t count, int msg_type, const float* d_llr,
+                                 uint8_t* d_bits, uint32_t* d_iterations, uint8_t* d_ok, void* stream) {
+    return guarded([&] {
+        std::unique_ptr<cf_batch_plan> p(create_plan(codes, llr_offsets, max_iterations, count, msg_type));
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        run_plan(*p, d_llr, d_bits, d_iterations, d_ok, false, s);
+        // the plan's tables must outlive the launch
+        check_hip(hipStreamSynchronize(s), "batch decode");
+    });
+}
+
+}  // extern "C"

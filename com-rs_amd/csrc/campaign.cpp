@@ -197,8 +197,10 @@ class NrLinkScenario : public sim::BatchScenario {
           llr_(size_t(cap_) * h->code.e), info_(size_t(cap_) * ((h->code.k_info + 31) / 32)),
           bits_(size_t(cap_) * h->code.k_info), its_(cap_), counters_(4) {
         check_hip(hipStreamCreate(&stream_), "hipStreamCreate");
-        if (link_.d_learned_params && !fast_layered(h_))
+        if (link_.d_learned_params && !layered_supported(h_->code.bg, int(h_->code.z)))
             fail(Kind::Config, "the learned decoder runs the layered min-sum schedule on a supported lifting size");
+        if (link_.d_learned_params && link_.fallback && !fast_layered(h_))
+            fail(Kind::Config, "the fallback runs the code's layered normalized or offset min-sum");
     }
     ~NrLinkScenario() override { (void)hipStreamDestroy(stream_); }
     std::string name() const override { return "nr_ldpc"; }
@@ -218,9 +220,17 @@ class NrLinkScenario : public sim::BatchScenario {
             g.llr = llr_.p;
             g.info = info_.p;
             check_hip(launch_generate(g, stream_), "generator launch");
-            if (fast_layered(h_)) {
-                run_layered(h_, device_, llr_.p, int64_t(n), bits_.p, nullptr, its_.p, link_.d_learned_params,
-                            stream_);
+            if (fast_layered(h_) || link_.d_learned_params) {
+                LayeredRun r;
+                r.llr = llr_.p;
+                r.count = int64_t(n);
+                r.bits = bits_.p;
+                r.iterations = its_.p;
+                r.params = link_.d_learned_params;
+                r.learned_iterations = link_.learned_iterations ? link_.learned_iterations
+                                                                : h_->settings().max_iterations;
+                r.fallback = link_.d_learned_params && link_.fallback;
+                run_layered(h_, device_, r, stream_);
             } else {
                 check_hip(cf_decode_batch(h_, CF_MSG_F32, llr_.p, n, bits_.p, nullptr, its_.p, stream_) == CF_OK
                               ? hipSuccess

@@ -90,35 +90,6 @@ bool fast_layered(const cf_code* h) {
     return s.layered && (s.rule == 1 || s.rule == 2) && layered_supported(h->code.bg, int(h->code.z));
 }
 
-void run_layered(const cf_code* h, int device, const float* llr, int64_t count, uint8_t* bits, float* post,
-                 uint32_t* iterations, const float* params, hipStream_t stream) {
-    DecoderSettings s = h->settings();
-    const NrCode& c = h->code;
-    const NrCode::DeviceTables& t = c.device_tables(device);
-    LayeredArgs a{};
-    a.z = int32_t(c.z);
-    a.k = int32_t(c.k);
-    a.k_info = int32_t(c.k_info);
-    a.e = int32_t(c.e);
-    a.row_start = t.row_start;
-    a.col = t.col;
-    a.shift = t.shift;
-    a.shift32 = t.shift32;
-    a.src_start = t.src_start;
-    a.src = t.src;
-    // g(m) = a max(m - b, 0): (alpha, 0) normalized, (1, beta) offset min-sum
-    a.a = s.rule == 1 ? float(s.alpha) : 1.0f;
-    a.b = s.rule == 1 ? 0.0f : float(s.alpha);
-    a.params = params;
-    a.max_iterations = int(s.max_iterations);
-    a.llr = llr;
-    a.count = count;
-    a.bits = bits;
-    a.post_out = post;
-    a.iterations = iterations;
-    check_hip(launch_layered(a, c.bg, stream), "layered decode launch");
-}
-
 // One codeword from host f64 buffers, decoded on the GPU in f64.
 void decode_host_f64(const cf_code* h, const double* llr, bool full, const double* apriori, double* post_host,
                      uint8_t* bits_host, uint32_t* iterations) {
@@ -260,9 +231,17 @@ cf_status cf_decode_batch(const cf_code* h, int msg_type, const void* d_llr, siz
             a.post_out = static_cast<double*>(d_post);
             a.iterations = d_iterations;
             run_generic(a, device, s);
-        } else if (msg_type == CF_MSG_F32 && fast_layered(h)) {
-            run_layered(h, device, static_cast<const float*>(d_llr), int64_t(count), d_bits,
-                        static_cast<float*>(d_post), d_iterations, nullptr, s);
+        } else if ((msg_type == CF_MSG_F32 || msg_type == CF_MSG_F16) && fast_layered(h)) {
+            LayeredRun r;
+            r.llr = static_cast<const float*>(d_llr);
+            r.count = int64_t(count);
+            r.bits = d_bits;
+            r.post = static_cast<float*>(d_post);
+            r.iterations = d_iterations;
+            r.half = msg_type == CF_MSG_F16;
+            run_layered(h, device, r, s);
+        } else if (msg_type == CF_MSG_F16) {
+            fail(Kind::Config, "FP16 storage runs the layered normalized or offset min-sum");
         } else if (msg_type == CF_MSG_F32) {
             GenericArgs<float> a = generic_args<float>(h, device);
             a.llr = static_cast<const float*>(d_llr);
@@ -272,7 +251,8 @@ cf_status cf_decode_batch(const cf_code* h, int msg_type, const void* d_llr, siz
             a.iterations = d_iterations;
             run_generic(a, device, s);
         } else {
-            fail(Kind::Config, "message type must be CF_MSG_F64 or CF_MSG_F32, got " + std::to_string(msg_type));
+            fail(Kind::Config, "message type must be CF_MSG_F64, CF_MSG_F32 or CF_MSG_F16, got " +
+                                   std::to_string(msg_type));
         }
     });
 }

@@ -39,8 +39,21 @@ cf_status status_of(Kind k);
 void check_hip(hipError_t err, const char* what);
 int current_device();
 bool fast_layered(const cf_code* h);
-void run_layered(const cf_code* h, int device, const float* llr, int64_t count, uint8_t* bits, float* post,
-                 uint32_t* iterations, const float* params, hipStream_t stream);
+// a launch of the layered min-sum kernel on `count` codewords of one code
+struct LayeredRun {
+    const float* llr = nullptr;
+    int64_t count = 0;
+    uint8_t* bits = nullptr;
+    float* post = nullptr;
+    uint32_t* iterations = nullptr;
+    uint8_t* ok = nullptr;
+    const float* params = nullptr;  // learned (a, b) per iteration and base row
+    uint32_t learned_iterations = 0;
+    bool fallback = false;
+    bool half = false;
+};
+void run_layered(const cf_code* h, int device, const LayeredRun& r, hipStream_t stream);
+BatchClass batch_class(const NrCode& c, const DecoderSettings& s, int device, int g);
 
 // Runs f, mapping a thrown Error to its status and the thread's last message.
 template <typename F>

@@ -1,0 +1,24 @@
+// The layered min-sum kernels with FP16 storage (one translation unit per
+// storage type, so that the builds run side by side).
+#include "decode_layered.hpp"
+
+namespace comfec {
+
+const void* layered_kernel_f16(int bg_mask, bool solo, int zc) {
+    using namespace layered;
+    if (solo && zc == 384) {
+        if (bg_mask == 1) return reinterpret_cast<const void*>(decode_layered_kernel<1, true, true, 384>);
+        if (bg_mask == 2) return reinterpret_cast<const void*>(decode_layered_kernel<2, true, true, 384>);
+    }
+    switch (bg_mask | (solo ? 4 : 0)) {
+        case 1: return reinterpret_cast<const void*>(decode_layered_kernel<1, true, false, 0>);
+        case 2: return reinterpret_cast<const void*>(decode_layered_kernel<2, true, false, 0>);
+        case 3: return reinterpret_cast<const void*>(decode_layered_kernel<3, true, false, 0>);
+        case 5: return reinterpret_cast<const void*>(decode_layered_kernel<1, true, true, 0>);
+        case 6: return reinterpret_cast<const void*>(decode_layered_kernel<2, true, true, 0>);
+        case 7: return reinterpret_cast<const void*>(decode_layered_kernel<3, true, true, 0>);
+        default: return nullptr;
+    }
+}
+
+}  // namespace comfec

@@ -40,32 +40,66 @@ struct GenericArgs {
     size_t scratch_stride;    // elements per workgroup
 };
 
-// The fast layered min-sum kernel (FP32 messages, check-to-variable messages in registers).
+// The layered min-sum kernel (messages in registers, posteriors in LDS).
+constexpr int kMaxThreads = 384;               // lanes of a workgroup: one per row of a layer
+constexpr int kMaxGroup = kMaxThreads / 2;     // codewords of one task (the smallest Z is 2)
+
+// One code of a decode batch, as the layered kernel sees it.
+struct BatchClass {
+    int32_t bg, z, n, k, k_info, e;
+    int32_t g;                  // codewords per task
+    int32_t max_iterations;     // of the code's decoder settings
+    float a, b;                 // its rule: g(m) = a max(m - b, 0)
+    const uint32_t* shift32;    // edges_b shifts V mod Z, layer order
+    const uint16_t* col;        // edges_b base columns
+    const uint16_t* row_start;  // rows_b + 1
+    const uint32_t* src_start;  // n + 1: rate recovery (the output bits of every codeword bit)
+    const uint32_t* src;        // e
+};
+
+// A workgroup's unit of work: codewords order[first .. first + count) of class `cls`.
+struct BatchTask {
+    uint32_t cls, first, count, unused;
+};
+
 struct LayeredArgs {
-    int32_t z, k, k_info, e;
-    const uint16_t* row_start;  // base tables (device)
-    const uint16_t* col;
-    const uint16_t* shift;
-    const uint32_t* shift32;    // the same shifts as 32-bit words
-    const uint32_t* src_start;
-    const uint32_t* src;
-    float a, b;                 // g(m) = a * max(m - b, 0)
-    const float* params;        // learned: (a, b) per iteration and layer, or null
-    int max_iterations;
+    BatchClass one;             // the class of a single-code batch (tasks == null)
+    const BatchClass* classes;  // per class, or null
+    const BatchTask* tasks;     // per task, or null: task t is codewords [t g, t g + g) of `one`
+    int32_t n_tasks;
+    uint32_t* queue;            // task counter, zero at launch (a fill kernel), or null: task t in block t mod grid
+    int64_t count;              // codewords (tasks == null)
+    const uint32_t* order;      // codeword ids in task order, or null (identity)
+    const uint64_t* llr_off;    // first LLR of each codeword, or null: id e
+    const uint64_t* bit_off;    // first decision of each codeword, or null: id K'
+    const uint16_t* max_it;     // iteration limit of each codeword, or null: the class's
+    const float* params;        // learned (a, b) per iteration and base row, or null
+    int32_t learned_iterations;
+    int32_t fallback;           // decode a failed learned codeword again with the class's rule
     const float* llr;
-    int64_t count;
     uint8_t* bits;
-    float* post_out;
-    uint32_t* iterations;
-    uint8_t* ok;                // per codeword: the syndrome holds at the end, or null
-    const uint32_t* index;      // decode codewords index[0 .. *index_count) only, or null
-    const uint32_t* index_count;
-    int add_iterations;         // add to iterations[] instead of storing
+    float* post_out;            // n per codeword id (single-code batches), or null
+    uint32_t* iterations;       // per codeword id, or null
+    uint8_t* ok;                // per codeword id: the syndrome held (learned: before any fallback), or null
+};
+
+struct LayeredLaunch {
+    int bg_mask;                // 1: base graph 1 only, 2: base graph 2 only, 3: both
+    bool half;                  // FP16 storage
+    bool solo;                  // every class has G = 1
+    int zc;                     // the Z of every class, when the kernel has it built in (384), or 0
+    int threads;                // a multiple of 64, at most kMaxThreads, at least every class's Z
+    int grid;
+    size_t lds;                 // the largest class's layered_lds_bytes
 };
 
 bool layered_supported(int bg, int z);
-// the codewords whose syndrome failed: out[0 .. *n) (n zeroed by the caller's fill kernel)
-hipError_t launch_failed_list(const uint8_t* ok, int64_t count, uint32_t* out, uint32_t* n, hipStream_t stream);
+// dynamic LDS of a task of g codewords
+size_t layered_lds_bytes(int bg, int z, int g, bool half);
+hipError_t layered_occupancy(int bg_mask, bool half, bool solo, int zc, int threads, size_t lds, int* blocks_per_cu);
+// sets the kernel's dynamic LDS limit for the launch (not while a stream is being captured)
+hipError_t layered_prepare(const LayeredLaunch& l);
+hipError_t launch_layered(const LayeredArgs& a, const LayeredLaunch& l, hipStream_t stream, bool prepare);
 
 // Training of the learned offset/scale min-sum (T unrolled layered iterations).
 struct LearnArgs {
@@ -89,7 +123,6 @@ hipError_t launch_learn_forward(const LearnArgs& a, hipStream_t stream);
 hipError_t launch_learn_backward(const LearnArgs& a, hipStream_t stream);
 hipError_t launch_adam(float* params, const float* grad, float* m, float* v, int n, float lr, float b1, float b2,
                        float eps, int step, hipStream_t stream);
-hipError_t launch_layered(const LayeredArgs& a, int bg, hipStream_t stream);
 
 // The device trial generator: information bits, encoder, rate matching, channel, LLRs.
 struct TrialGenArgs {

@@ -1,7 +1,7 @@
 // The C ABI of the learned offset/scale min-sum (include/comfec_gpu.h): training batches
 // from the device trial generator, the loss and its gradient through the unrolled
 // iterations, Adam, a training loop, and decoding with the learned parameters, codewords
-// whose syndrome still fails being decoded again by the plain min-sum.
+// whose syndrome still fails being decoded again by the plain min-sum in the same launch.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -18,7 +18,7 @@ using namespace comfec::detail;
 
 namespace {
 
-enum { kTrainSlot = 1, kFallbackSlot = 2 };
+enum { kTrainSlot = 1 };
 
 struct TrainBuffers {
     float* ckpt;
@@ -120,57 +120,27 @@ cf_status cf_learned_decode_batch(const cf_code* h, uint32_t iterations, const f
                                   uint8_t* d_ok, void* stream) {
     return guarded([&] {
         if (!h || !d_params || !d_llr) fail(Kind::Config, "null handle or buffer");
+        if (iterations < 1) fail(Kind::Config, "the learned decoder needs at least one iteration");
         if (count == 0) return;
         if (!layered_supported(h->code.bg, int(h->code.z)))
             fail(Kind::Config, "the learned decoder needs a lifting size of at most 384");
-        const int device = current_device();
-        hipStream_t s = static_cast<hipStream_t>(stream);
-        WorkspaceSlot& slot = workspace(device, s, kFallbackSlot);
-        std::lock_guard<std::mutex> l(slot.mu);
-        // ok flags, the failed list and its count, iteration counts if the caller has none
-        auto up64 = [](size_t x) { return (x + 63) & ~size_t(63); };
-        const size_t at_list = up64(count), at_n = up64(at_list + 4 * count), at_its = at_n + 64;
-        uint8_t* base = static_cast<uint8_t*>(slot.ws.get(at_its + 4 * count));
-        if (!base) fail(Kind::Device, "out of device memory");
-        uint8_t* ok = d_ok ? d_ok : base;
-        uint32_t* list = reinterpret_cast<uint32_t*>(base + at_list);
-        uint32_t* n_fail = reinterpret_cast<uint32_t*>(base + at_n);  // 8-byte aligned: zeroed as a u64
-        uint32_t* its = d_iterations ? d_iterations : reinterpret_cast<uint32_t*>(base + at_its);
-        const NrCode& c = h->code;
-        const NrCode::DeviceTables& t = c.device_tables(device);
-        LayeredArgs a{};
-        a.z = int32_t(c.z);
-        a.k = int32_t(c.k);
-        a.k_info = int32_t(c.k_info);
-        a.e = int32_t(c.e);
-        a.row_start = t.row_start;
-        a.col = t.col;
-        a.shift = t.shift;
-        a.shift32 = t.shift32;
-        a.src_start = t.src_start;
-        a.src = t.src;
-        a.params = d_params;
-        a.max_iterations = int(iterations);
-        a.llr = d_llr;
-        a.count = int64_t(count);
-        a.bits = d_bits;
-        a.iterations = its;
-        a.ok = ok;
-        check_hip(launch_layered(a, c.bg, s), "learned decode launch");
-        if (!fallback) return;
-        // the plain min-sum of the code's settings on the codewords whose syndrome failed
-        check_hip(launch_zero(reinterpret_cast<uint64_t*>(n_fail), 1, s), "zero launch");
-        check_hip(launch_failed_list(ok, int64_t(count), list, n_fail, s), "list launch");
-        DecoderSettings set = h->settings();
-        a.params = nullptr;
-        a.a = set.rule == 1 ? float(set.alpha) : 1.0f;
-        a.b = set.rule == 1 ? 0.0f : float(set.alpha);
-        a.max_iterations = int(set.max_iterations);
-        a.index = list;
-        a.index_count = n_fail;
-        a.add_iterations = 1;
-        a.ok = nullptr;  // ok keeps the learned decoder's own verdict
-        check_hip(launch_layered(a, c.bg, s), "fallback decode launch");
+        if (fallback) {
+            const DecoderSettings s = h->settings();
+            if (!s.layered || (s.rule != 1 && s.rule != 2))
+                fail(Kind::Config, "the fallback runs the code's layered normalized or offset min-sum");
+        }
+        // one launch: the learned iterations, then (fallback) the plain min-sum of the
+        // code's settings on the codewords whose syndrome still fails, in the same workgroup
+        LayeredRun r;
+        r.llr = d_llr;
+        r.count = int64_t(count);
+        r.bits = d_bits;
+        r.iterations = d_iterations;
+        r.ok = d_ok;
+        r.params = d_params;
+        r.learned_iterations = iterations;
+        r.fallback = fallback != 0;
+        run_layered(h, current_device(), r, static_cast<hipStream_t>(stream));
     });
 }
 

@@ -175,11 +175,6 @@ __global__ void __launch_bounds__(256) count_kernel(CountArgs a) {
     }
 }
 
-__global__ void failed_list_kernel(const uint8_t* ok, int64_t count, uint32_t* out, uint32_t* n) {
-    const int64_t w = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (w < count && !ok[w]) out[atomicAdd(n, 1u)] = uint32_t(w);
-}
-
 __global__ void zero_kernel(uint64_t* p, int n) {
     const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
     if (i < n) p[i] = 0;
@@ -190,13 +185,6 @@ __global__ void zero_kernel(uint64_t* p, int n) {
 hipError_t launch_zero(uint64_t* p, int n, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(zero_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream, p, n);
-    return hipGetLastError();
-}
-
-hipError_t launch_failed_list(const uint8_t* ok, int64_t count, uint32_t* out, uint32_t* n, hipStream_t stream) {
-    if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(failed_list_kernel, dim3(unsigned((count + 255) / 256)), dim3(256), 0, stream, ok, count, out,
-                       n);
     return hipGetLastError();
 }
 

@@ -41,8 +41,9 @@ typedef enum {
 enum { CF_RULE_SUM_PRODUCT = 0, CF_RULE_NORMALIZED_MIN_SUM = 1, CF_RULE_OFFSET_MIN_SUM = 2 };
 
 /* Message arithmetic of the device decoder. F64 reproduces the reference's f64 decoder
-   bit for bit (min-sum rules); F32 keeps FP32 messages; F16 stores the check-to-variable
-   messages in FP16 and computes in FP32. */
+   bit for bit (min-sum rules); F32 keeps FP32 messages; F16 (layered min-sum rules) stores
+   the posteriors and the check-to-variable magnitudes in binary16 (round to nearest even,
+   clamped to +-65504, a zero posterior stored as +0.0) and computes in FP32. */
 enum { CF_MSG_F64 = 0, CF_MSG_F32 = 1, CF_MSG_F16 = 2 };
 
 typedef struct {
@@ -113,6 +114,61 @@ cf_status cf_app_into(const cf_code* code, const double* llr_host, size_t e,
    input was a codeword). Asynchronous on `stream`. */
 cf_status cf_decode_batch(const cf_code* code, int msg_type, const void* d_llr, size_t count,
                           uint8_t* d_bits, void* d_post, uint32_t* d_iterations, void* stream);
+
+
+/* ---- batches of many codes: one persistent launch --------------------------------------- */
+
+/* A decode plan for a fixed list of codewords of possibly different codes (any mix of the
+   two base graphs and of lifting sizes up to 384, each codeword with its own iteration
+   limit): BlockCode::decode_block_into (crates/com-fec/src/block.rs:14-90) over a stream of
+   heterogeneous blocks, as the runner's trial bodies call it (crates/com-sim/src/
+   runner.rs:435-446). The codes' decoder settings (layered normalized or offset min-sum) are
+   read when the plan is made, and the handles must outlive the plan. Codeword i has the
+   E_i = llr_offsets[i+1] - llr_offsets[i] f32 LLRs d_llr[llr_offsets[i] ..]; its K'_i
+   decisions go to d_bits at the sum of the K'_j of the codewords before it; d_iterations
+   and d_ok (may be NULL) get one entry per codeword. max_iterations: per codeword, or NULL
+   for each code's own limit. One launch decodes the whole list: workgroups take tasks (up
+   to floor(threads / Z) codewords of one code) from a counter, the most work first.
+   msg_type: CF_MSG_F32 or CF_MSG_F16. A plan is used by one thread at a time. */
+typedef struct cf_batch_plan cf_batch_plan;
+
+typedef struct {
+    uint64_t llr_total;     /* llr_offsets[count] */
+    uint64_t bits_total;    /* the sum of K' */
+    uint32_t codewords, classes, tasks;
+    uint32_t threads;       /* lanes per workgroup */
+    uint32_t grid;          /* workgroups of the launch (resident ones: the launch is persistent) */
+    uint32_t lds_bytes;     /* dynamic LDS per workgroup */
+    int msg_type;
+} cf_batch_plan_dims;
+
+cf_status cf_batch_plan_create(const cf_code* const* codes, const uint64_t* llr_offsets,
+                               const uint32_t* max_iterations, size_t count, int msg_type,
+                               cf_batch_plan** out);
+void cf_batch_plan_destroy(cf_batch_plan* plan);
+cf_status cf_batch_plan_dims_get(const cf_batch_plan* plan, cf_batch_plan_dims* out);
+
+/* The workgroup width: a multiple of 64 from the largest Z (rounded up) to 384. The
+   results do not depend on it. Waits for the device. */
+cf_status cf_batch_plan_set_threads(cf_batch_plan* plan, uint32_t threads);
+
+/* The autotuner: times one launch of every admissible width on these buffers (repeats
+   times each, after a warm-up) and keeps the fastest; *best_ms (may be NULL) gets its
+   time. The outputs hold a complete decode afterwards. */
+cf_status cf_batch_plan_tune(cf_batch_plan* plan, const float* d_llr, uint8_t* d_bits,
+                             uint32_t* d_iterations, uint8_t* d_ok, uint32_t repeats,
+                             double* best_ms, void* stream);
+
+/* Decode on `stream`. With use_graph the counter fill and the launch are captured once as a
+   HIP graph and replayed while the buffers stay the same. */
+cf_status cf_batch_plan_run(cf_batch_plan* plan, const float* d_llr, uint8_t* d_bits,
+                            uint32_t* d_iterations, uint8_t* d_ok, int use_graph, void* stream);
+
+/* A plan made, run once without a graph and freed (waits for the stream). */
+cf_status cf_decode_batch_uneven(const cf_code* const* codes, const uint64_t* llr_offsets,
+                                 const uint32_t* max_iterations, size_t count, int msg_type,
+                                 const float* d_llr, uint8_t* d_bits, uint32_t* d_iterations,
+                                 uint8_t* d_ok, void* stream);
 
 
 /* ---- the device trial generator ------------------------------------------------------ */
@@ -193,6 +249,8 @@ typedef struct {
     int modulation;               /* CF_BPSK, CF_QPSK, CF_QAM16, CF_QAM64 */
     int snr_kind;                 /* CF_EBN0_DB or CF_ESN0_DB */
     const float* d_learned_params;
+    uint32_t learned_iterations;  /* iterations of the learned parameters (0: the code's limit) */
+    int fallback;                 /* learned: decode failed codewords again with the code's rule */
 } cf_nr_link;
 
 /* Runner::run with the NR link as the scenario: batches of `batch` trials on the GPU. */

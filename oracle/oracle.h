@@ -70,6 +70,15 @@ int or_decode_llr_f32(const or_code* c, const or_dec* d, const float* llr, float
 void or_decode_many_f32(const or_code* c, const or_dec* d, const float* llr, int64_t count,
                         float* post, int32_t* iterations, int threads);
 
+/* the layered min-sum (rules NMS and OMS) with the posteriors and check-to-variable
+   messages stored in IEEE binary16 and the arithmetic in FP32 (the device FP16 mode);
+   max_iterations, when not null, overrides d->max_iterations per codeword */
+uint16_t or_f32_to_f16(float x);  /* round to nearest even */
+float or_f16_to_f32(uint16_t h);
+int or_decode_llr_f16(const or_code* c, const or_dec* d, const float* llr, float* post);
+void or_decode_many_f16(const or_code* c, const or_dec* d, const float* llr, int64_t count,
+                        const int32_t* max_iterations, float* post, int32_t* iterations, int threads);
+
 /* learned offset/scale min-sum (no reference counterpart): layered schedule, iteration t
    and base row i use g(m) = alpha[t][i] * max(m - beta[t][i], 0); params t-major. */
 int or_learned_decode(const or_code* c, int iterations, const double* alpha, const double* beta,

@@ -66,6 +66,11 @@ def _load():
         "or_decode_many": (None, [C.c_void_p, C.POINTER(OrDec), _f64p, C.c_int64, _f64p, _i32p, C.c_int]),
         "or_decode_llr_f32": (C.c_int, [C.c_void_p, C.POINTER(OrDec), _f32p, _f32p]),
         "or_decode_many_f32": (None, [C.c_void_p, C.POINTER(OrDec), _f32p, C.c_int64, _f32p, _i32p, C.c_int]),
+        "or_f32_to_f16": (C.c_uint16, [C.c_float]),
+        "or_f16_to_f32": (C.c_float, [C.c_uint16]),
+        "or_decode_llr_f16": (C.c_int, [C.c_void_p, C.POINTER(OrDec), _f32p, _f32p]),
+        "or_decode_many_f16": (None, [C.c_void_p, C.POINTER(OrDec), _f32p, C.c_int64, C.c_void_p, _f32p, _i32p,
+                                      C.c_int]),
         "or_db2lin": (C.c_double, [C.c_double]),
         "or_bpsk_noise_variance": (C.c_double, [C.c_double, C.c_double]),
         "or_ebn0_noise_variance": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_double]),
@@ -216,6 +221,24 @@ class Code:
         lib.or_decode_many_f32(self.h, C.byref(d), llr, len(llr), post, its, threads)
         return post, its
 
+    def decode_many_f16(self, llr, rule=NMS, alpha=0.75, max_iterations=20, threads=1):
+        """The layered min-sum with the messages stored in binary16 and FP32 arithmetic.
+        max_iterations: one limit, or one per codeword. Returns the posteriors (the stored
+        halves widened to float32) and the iterations."""
+        llr = np.ascontiguousarray(llr, np.float32).reshape(-1, self.n)
+        per = None
+        lim = max_iterations
+        if np.ndim(max_iterations):
+            per = np.ascontiguousarray(max_iterations, np.int32)
+            assert per.shape == (len(llr),)
+            lim = 0
+        d = OrDec(rule, alpha, 1, int(lim))
+        post = np.zeros_like(llr)
+        its = np.zeros(len(llr), np.int32)
+        lib.or_decode_many_f16(self.h, C.byref(d), llr, len(llr),
+                               None if per is None else per.ctypes.data_as(C.c_void_p), post, its, threads)
+        return post, its
+
     def trial_bpsk_lifted(self, ebn0_db, seed, name, cond, trial):
         cw = np.zeros(self.n, np.uint8)
         llr = np.zeros(self.n)
@@ -245,6 +268,15 @@ class Code:
                                         np.ascontiguousarray(llr, np.float64),
                                         np.ascontiguousarray(cw, np.uint8), ga.ravel(), gb.ravel())
         return loss, ga, gb
+
+
+def f32_to_f16_bits(x):
+    """binary32 -> binary16 bits, round to nearest even (the C restatement)."""
+    return np.array([lib.or_f32_to_f16(float(v)) for v in np.asarray(x, np.float32).ravel()], np.uint16)
+
+
+def f16_bits_to_f32(h):
+    return np.array([lib.or_f16_to_f32(int(v)) for v in np.asarray(h, np.uint16).ravel()], np.float32)
 
 
 def wilson(errors, total, confidence=0.95):
