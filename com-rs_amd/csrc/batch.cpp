@@ -74,8 +74,9 @@ void run_layered(const cf_code* h, int device, const LayeredRun& r, hipStream_t 
     a.post_out = r.post;
     a.iterations = r.iterations;
     a.ok = r.ok;
-    LayeredLaunch l{c.bg, r.half, g == 1, c.z == 384 && g == 1 ? 384 : 0, threads, int(a.n_tasks),
-                    layered_lds_bytes(c.bg, int(c.z), g, r.half)};
+    const size_t off = layered_shared_offset(layered_lds_bytes(c.bg, int(c.z), g, r.half));
+    a.shared_off = int32_t(off);
+    LayeredLaunch l{c.bg, r.half, g == 1, c.z == 384 && g == 1 ? 384 : 0, threads, int(a.n_tasks), off + kSharedBytes};
     check_hip(launch_layered(a, l, stream, true), "layered decode launch");
 }
 
@@ -152,6 +153,8 @@ void configure(cf_batch_plan& p, int threads) {
         all384 = all384 && c.z == 384;
     }
     const int zc = solo && all384 && p.bg_mask != 3 ? 384 : 0;
+    const size_t shared_off = layered_shared_offset(lds);
+    lds = shared_off + kSharedBytes;
     std::vector<std::vector<uint32_t>> members(nc);
     for (size_t i = 0; i < p.count; ++i) members[p.cls_of[i]].push_back(uint32_t(i));
     struct Pending {
@@ -223,6 +226,7 @@ void configure(cf_batch_plan& p, int threads) {
     a.llr_off = reinterpret_cast<const uint64_t*>(d + o_llr);
     a.bit_off = reinterpret_cast<const uint64_t*>(d + o_bit);
     a.max_it = reinterpret_cast<const uint16_t*>(d + o_it);
+    a.shared_off = int32_t(shared_off);
 }
 
 // the counter fill and the decode launch

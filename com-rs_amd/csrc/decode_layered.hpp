@@ -93,10 +93,30 @@ constexpr bool barrier_before(int layer) {
     return false;
 }
 
+// The lifting set of Z and the shift V mod Z of base entry e, when Z is known at compile
+// time (nr/ldpc.rs:117-133).
+constexpr int lifting_set_of(int z) {
+    for (int s = 0; s < 8; ++s)
+        for (int k = 0; k < NR_LIFTING_SET_MAX; ++k)
+            if (NR_LIFTING_SETS[s][k] == z) return s;
+    return -1;
+}
+
+template <int BG, int ZC>
+constexpr uint32_t shift_of(int e) {
+    return uint32_t(Graph<BG>::t[e].v[lifting_set_of(ZC)] % ZC);
+}
+
 // ---- storage of the posteriors and of the message magnitudes ----------------------------
 
 template <bool HALF>
 struct Store;
+
+// a pointer into LDS made from a byte address
+template <typename T>
+struct lds_ptr {
+    typedef __attribute__((address_space(3))) T* type;
+};
 
 template <>
 struct Store<false> {
@@ -104,6 +124,9 @@ struct Store<false> {
     static __device__ __forceinline__ float ld(const float* p, int i) { return p[i]; }
     static __device__ __forceinline__ void st(float* p, int i, float x) { p[i] = x; }
     static __device__ __forceinline__ uint32_t sign(const float* p, int i) { return __float_as_uint(p[i]) >> 31; }
+    // at LDS byte address `off` (the kernel's posteriors start at LDS address 0)
+    static __device__ __forceinline__ float ld_at(uint32_t off) { return *(const lds_ptr<float>::type)(uintptr_t)off; }
+    static __device__ __forceinline__ void st_at(uint32_t off, float x) { *(lds_ptr<float>::type)(uintptr_t)off = x; }
 };
 
 __device__ __forceinline__ float h2f(uint32_t h) {
@@ -122,6 +145,13 @@ struct Store<true> {
         p[i] = uint16_t((h & 0x7fffu) ? h : 0u);  // a zero is stored as +0.0
     }
     static __device__ __forceinline__ uint32_t sign(const uint16_t* p, int i) { return uint32_t(p[i]) >> 15; }
+    static __device__ __forceinline__ float ld_at(uint32_t off) {
+        return h2f(*(const lds_ptr<uint16_t>::type)(uintptr_t)off);
+    }
+    static __device__ __forceinline__ void st_at(uint32_t off, float x) {
+        const uint32_t h = f2h(__builtin_amdgcn_fmed3f(x, -65504.0f, 65504.0f));
+        *(lds_ptr<uint16_t>::type)(uintptr_t)off = uint16_t((h & 0x7fffu) ? h : 0u);
+    }
 };
 
 // The messages of one lane's rows: output magnitudes g(m1), g(m2) and, packed, the
@@ -205,7 +235,10 @@ __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post
     // With Z known at compile time (ZC) the column bases are immediate offsets.
     uint32_t base = 0;
     if constexpr (ZC != 0) {
+        // the shifts are constants too, so the row index is what keeps the addresses (all
+        // 316 of them loop-invariant) from being computed once and held in registers
         z = ZC;
+        asm volatile("" : "+v"(r));
     } else {
         asm volatile("" : "+s"(z));
     }
@@ -218,18 +251,28 @@ __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post
     const uint32_t pko = msg.pk[I];
     const uint32_t i1o = pko >> 24;
     float v2c[deg];
-    int addr[deg];
+    int addr[deg];  // element index; with ZC, the LDS byte address (the posteriors start at 0)
     float m1 = INFINITY, m2 = INFINITY;
     uint32_t parity = 0;
+    constexpr uint32_t es = sizeof(typename S::T);
 #pragma unroll
     for (int q = 0; q < deg; ++q) {
         // (r + shift) mod z: the unsigned min of t and t - z (which wraps when t < z)
-        const uint32_t t = uint32_t(r) + shift[e0 + q];
-        const uint32_t rr = min(t, t - uint32_t(z));
-        addr[q] = int(Graph<BG>::t[e0 + q].j) * z + int(rr) + int(base);
+        if constexpr (ZC != 0) {
+            // in bytes, so that the column base is the LDS instruction's immediate offset; a
+            // zero shift (every extension column) needs no wrap
+            const uint32_t s_q = shift_of<BG, ZC>(e0 + q) * es;
+            const uint32_t t = uint32_t(r) * es + s_q;
+            const uint32_t rr = s_q == 0 ? t : min(t, t - uint32_t(ZC) * es);
+            addr[q] = int(rr) + int(Graph<BG>::t[e0 + q].j) * ZC * int(es);
+        } else {
+            const uint32_t t = uint32_t(r) + shift[e0 + q];
+            const uint32_t rr = min(t, t - uint32_t(z));
+            addr[q] = int(Graph<BG>::t[e0 + q].j) * z + int(rr) + int(base);
+        }
         const float mag = uint32_t(q) == i1o ? g2o : g1o;
         const float old = __uint_as_float(__float_as_uint(mag) ^ ((pko << (32 - deg + q)) & 0x80000000u));
-        const float in = S::ld(post, addr[q]) - old;
+        const float in = (ZC != 0 ? S::ld_at(uint32_t(addr[q])) : S::ld(post, addr[q])) - old;
         v2c[q] = in;
         parity ^= __float_as_uint(in);
         const float m = fabsf(in);
@@ -249,7 +292,8 @@ __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post
         const uint32_t sb = __float_as_uint(v2c[q]) ^ parity;
         signs = __builtin_amdgcn_alignbit(signs, sb, 31u);
         const float out = __uint_as_float(__float_as_uint(mag) ^ (sb & 0x80000000u));
-        S::st(post, addr[q], v2c[q] + out);
+        if constexpr (ZC != 0) S::st_at(uint32_t(addr[q]), v2c[q] + out);
+        else S::st(post, addr[q], v2c[q] + out);
     }
     msg.pk[I] = signs | (i1 << 24);
 }
@@ -276,7 +320,52 @@ struct Shared {
     uint16_t row_start[48];
     uint32_t fails[kMaxGroup];  // per codeword of the task: the syndrome stamp of a failing check
     uint32_t task;
+    uint32_t vote[2];           // "some codeword still runs", alternate rounds
 };
+
+// Whether any lane of the workgroup has `pred` (one barrier). Two flags in turn: past the
+// barrier every lane has finished the round before, so the other flag can be cleared for
+// the round after, which the caller reaches only through further barriers.
+__device__ __forceinline__ bool any_lane(Shared& sh, bool pred, uint32_t& round) {
+    const uint32_t slot = round & 1u;
+    ++round;
+    if (pred) sh.vote[slot] = 1u;
+    __syncthreads();
+    if (threadIdx.x == 0) sh.vote[slot ^ 1u] = 0u;
+    return sh.vote[slot] != 0u;
+}
+static_assert(sizeof(Shared) <= kSharedBytes, "the tables outgrow their LDS slot");
+
+// The signs of 32 consecutive posteriors (16-byte aligned) as one word, bit k = posterior k:
+// each funnel shift moves the word up one bit and brings in a sign bit at bit 0.
+template <bool HALF>
+__device__ __forceinline__ uint32_t pack_signs(const typename Store<HALF>::T* p) {
+    const uint4* v = reinterpret_cast<const uint4*>(p);
+    uint32_t acc = 0;
+    if constexpr (!HALF) {
+#pragma unroll
+        for (int k = 7; k >= 0; --k) {
+            const uint4 x = v[k];
+            acc = __builtin_amdgcn_alignbit(acc, x.w, 31u);
+            acc = __builtin_amdgcn_alignbit(acc, x.z, 31u);
+            acc = __builtin_amdgcn_alignbit(acc, x.y, 31u);
+            acc = __builtin_amdgcn_alignbit(acc, x.x, 31u);
+        }
+    } else {
+        // two halves a word: the high one's sign is bit 31, the low one's bit 15
+#pragma unroll
+        for (int k = 3; k >= 0; --k) {
+            const uint4 x = v[k];
+            const uint32_t d[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+            for (int i = 3; i >= 0; --i) {
+                acc = __builtin_amdgcn_alignbit(acc, d[i], 31u);
+                acc = __builtin_amdgcn_alignbit(acc, d[i] << 16, 31u);
+            }
+        }
+    }
+    return acc;
+}
 
 // Whether the syndrome of the lane's codeword fails (ldpc.rs:555-557). Every codeword of
 // the task is checked; a failing codeword has fails[g] == stamp afterwards.
@@ -285,20 +374,16 @@ __device__ __forceinline__ bool syndrome_fails(const typename Store<HALF>::T* po
                                uint32_t stamp, int z, int n, int cnt, int g, int r, bool lane_ok) {
     using S = Store<HALF>;
     if ((z & 31) == 0) {
-        // packed: ballots of 64 decisions into 32-bit words, each column's words doubled
+        // packed: every lane packs the signs of 32 consecutive posteriors (one word of a
+        // column) with funnel shifts, bit k = posterior k; each column's words doubled
         const int w32 = z >> 5, words = n >> 5, per_cw = Graph<BG>::cols * 2 * w32;
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
-        for (int c = wave; c < ((cnt * n) >> 6); c += waves) {
-            const uint64_t m = __ballot(S::sign(post_all, c * 64 + lane) != 0u);
-            if (lane < 2) {
-                const int f = 2 * c + lane;
-                const int gg = f / words, fw = f - gg * words;
-                const int j = fw / w32, w = fw - j * w32;
-                const uint32_t word = lane ? uint32_t(m >> 32) : uint32_t(m);
-                uint32_t* hd = hd_all + gg * per_cw + j * 2 * w32;
-                hd[w] = word;
-                hd[w32 + w] = word;
-            }
+        for (int f = threadIdx.x; f < cnt * words; f += blockDim.x) {
+            const int gg = f / words, fw = f - gg * words;
+            const int j = fw / w32, w = fw - j * w32;
+            const uint32_t word = pack_signs<HALF>(post_all + gg * n + fw * 32);
+            uint32_t* hd = hd_all + gg * per_cw + j * 2 * w32;
+            hd[w] = word;
+            hd[w32 + w] = word;
         }
         __syncthreads();
         const int per_task = Graph<BG>::rows * w32;
@@ -350,10 +435,10 @@ __device__ __forceinline__ void place_llrs(typename Store<HALF>::T* post, const 
 // the launch has the same (0: read at run time).
 template <int BG, bool HALF, bool SOLO, int ZC>
 __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass& c, uint32_t first, int cnt, Shared& sh,
-                         void* lds_dyn, uint32_t& stamp) {
+                         void* lds_dyn, uint32_t& stamp, uint32_t& round) {
     using S = Store<HALF>;
     using T = typename S::T;
-    const int z = ZC ? ZC : c.z, n = c.n;
+    const int z = ZC ? ZC : c.z, n = ZC ? Graph<BG>::cols * ZC : c.n;
     T* post_all = static_cast<T*>(lds_dyn);
     uint32_t* hd_all = reinterpret_cast<uint32_t*>(post_all + c.g * n);  // g n is even: 4-byte aligned
     const int tid = threadIdx.x;
@@ -400,7 +485,7 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
             finished = true;
             its = uint32_t(offset + it);
         }
-        if (!__syncthreads_or(!finished)) break;
+        if (!any_lane(sh, !finished, round)) break;
         ++it;
         const float* ab = learned ? a.params + size_t(it - 1) * 2 * Graph<BG>::rows : nullptr;
         sweep<BG, HALF, SOLO, ZC, 0>(post_all, r, uint32_t(g * n), z, !finished, sh.shift, msg, ab, c.a, c.b);
@@ -463,11 +548,14 @@ __device__ __forceinline__ BatchClass uni(const BatchClass& c) {
 // Workgroups take tasks until none is left. BGMASK: the base graphs of the classes; SOLO:
 // every class has G = 1; ZC: every class has Z = ZC (0: any).
 template <int BGMASK, bool HALF, bool SOLO, int ZC>
-__global__ void __launch_bounds__(kMaxThreads) decode_layered_kernel(LayeredArgs a) {
+__global__ void __launch_bounds__(kMaxThreads, HALF ? 3 : 1) decode_layered_kernel(LayeredArgs a) {
+    // one dynamic allocation and no static LDS, so that the posteriors start at address 0
+    // and the column bases are immediate offsets of the LDS instructions
     extern __shared__ __align__(16) uint32_t lds_dyn[];
-    __shared__ Shared sh;
+    Shared& sh = *reinterpret_cast<Shared*>(reinterpret_cast<char*>(lds_dyn) + a.shared_off);
     for (int i = threadIdx.x; i < kMaxGroup; i += blockDim.x) sh.fails[i] = 0u;
-    uint32_t stamp = 0;
+    if (threadIdx.x < 2) sh.vote[threadIdx.x] = 0u;
+    uint32_t stamp = 0, round = 0;
     int loaded = -1;
     uint32_t t = blockIdx.x;
     for (;;) {
@@ -507,10 +595,10 @@ __global__ void __launch_bounds__(kMaxThreads) decode_layered_kernel(LayeredArgs
             loaded = cls;
         }
         if constexpr ((BGMASK & 1) != 0) {
-            if (c.bg == 1) run_task<1, HALF, SOLO, ZC>(a, c, first, cnt, sh, lds_dyn, stamp);
+            if (c.bg == 1) run_task<1, HALF, SOLO, ZC>(a, c, first, cnt, sh, lds_dyn, stamp, round);
         }
         if constexpr ((BGMASK & 2) != 0) {
-            if (c.bg == 2) run_task<2, HALF, SOLO, ZC>(a, c, first, cnt, sh, lds_dyn, stamp);
+            if (c.bg == 2) run_task<2, HALF, SOLO, ZC>(a, c, first, cnt, sh, lds_dyn, stamp, round);
         }
         if (!a.queue) t += gridDim.x;
     }

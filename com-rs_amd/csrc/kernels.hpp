@@ -43,6 +43,7 @@ struct GenericArgs {
 // The layered min-sum kernel (messages in registers, posteriors in LDS).
 constexpr int kMaxThreads = 384;               // lanes of a workgroup: one per row of a layer
 constexpr int kMaxGroup = kMaxThreads / 2;     // codewords of one task (the smallest Z is 2)
+constexpr size_t kSharedBytes = 3072;          // a workgroup's tables and flags, after its posteriors
 
 // One code of a decode batch, as the layered kernel sees it.
 struct BatchClass {
@@ -81,6 +82,7 @@ struct LayeredArgs {
     float* post_out;            // n per codeword id (single-code batches), or null
     uint32_t* iterations;       // per codeword id, or null
     uint8_t* ok;                // per codeword id: the syndrome held (learned: before any fallback), or null
+    int32_t shared_off;         // byte offset of the tables in the dynamic LDS (after the posteriors)
 };
 
 struct LayeredLaunch {
@@ -90,12 +92,14 @@ struct LayeredLaunch {
     int zc;                     // the Z of every class, when the kernel has it built in (384), or 0
     int threads;                // a multiple of 64, at most kMaxThreads, at least every class's Z
     int grid;
-    size_t lds;                 // the largest class's layered_lds_bytes
+    size_t lds;                 // dynamic LDS: layered_shared_offset + kSharedBytes
 };
 
 bool layered_supported(int bg, int z);
-// dynamic LDS of a task of g codewords
+// LDS of a task of g codewords (posteriors and packed decisions)
 size_t layered_lds_bytes(int bg, int z, int g, bool half);
+// where the tables go: after the largest task's LDS, 16-byte aligned
+inline size_t layered_shared_offset(size_t task_lds) { return (task_lds + 15) & ~size_t(15); }
 hipError_t layered_occupancy(int bg_mask, bool half, bool solo, int zc, int threads, size_t lds, int* blocks_per_cu);
 // sets the kernel's dynamic LDS limit for the launch (not while a stream is being captured)
 hipError_t layered_prepare(const LayeredLaunch& l);

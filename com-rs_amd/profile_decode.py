@@ -1,8 +1,13 @@
-"""A short driver for profilers: decode one batch of the BG1 Z=384 16-QAM workload a few
-times with the plain or the learned decoder (or copy a buffer of known size, the yardstick
-of the HBM byte counters).
+"""A short driver for profilers and timings: decode one batch of the BG1 Z=384 16-QAM
+workload a few times with the plain or the learned decoder (FP32 or FP16 storage), or the
+uneven workload (codewords of every NR base graph and lifting size 16..384, BPSK/QPSK,
+Eb/N0 uniform in [0, 3] dB, iteration limits 5..25) through one batch plan, or copy a
+buffer of known size (the yardstick of the HBM byte counters). Prints the mean time of a
+step measured with HIP events.
 
     python com-rs_amd/profile_decode.py --decoder plain --batch 4096 --steps 5 --esn0 4.2
+    python com-rs_amd/profile_decode.py --decoder plain --half
+    python com-rs_amd/profile_decode.py --uneven 20000 --half --graph
     python com-rs_amd/profile_decode.py --copy-mib 1024 --steps 5
 
 Under rocprofv3, e.g.:
@@ -25,9 +30,15 @@ def main():
     p.add_argument("--iterations", type=int, default=8, help="learned decoder iterations")
     p.add_argument("--train-steps", type=int, default=200)
     p.add_argument("--copy-mib", type=int, default=0)
+    p.add_argument("--half", action="store_true", help="FP16 message storage")
+    p.add_argument("--uneven", type=int, default=0, help="codewords of the uneven workload")
+    p.add_argument("--graph", action="store_true", help="uneven: replay a captured HIP graph")
+    p.add_argument("--tune", action="store_true", help="uneven: autotune the workgroup width first")
     a = p.parse_args()
     import torch
     import comfec as F
+    if a.uneven:
+        return uneven(a, torch, F)
     if a.copy_mib:
         src = torch.empty(a.copy_mib << 18, dtype=torch.float32, device="cuda").fill_(1.0)
         dst = torch.empty_like(src)
@@ -43,15 +54,71 @@ def main():
         params = F.initial_params(code, a.iterations)
         F.learned_train(code, params, steps=a.train_steps, batch=512, lr=1e-2, modulation=F.QAM16,
                         snr_kind=F.ESN0_DB, snr_lo=a.esn0 - 0.1, snr_hi=a.esn0 + 0.3, seed=11)
-    its = None
-    for _ in range(a.steps):
+    def step():
         if params is None:
-            _, its, _ = code.decode_batch(llr)
-        else:
-            _, its, _ = F.learned_decode_batch(code, params, llr, fallback=True)
-    torch.cuda.synchronize()
-    print(f"{a.decoder}: {a.steps} x {a.batch} codewords, mean iterations {its.float().mean().item():.3f}",
+            return code.decode_batch(llr, half=a.half)[1]
+        return F.learned_decode_batch(code, params, llr, fallback=True)[1]
+
+    its = step()
+    ms = timed(step, a.steps, torch)
+    print(f"{a.decoder}{' fp16' if a.half else ''}: {a.steps} x {a.batch} codewords, mean iterations "
+          f"{its.float().mean().item():.3f}, {ms:.3f} ms per step, {8448 * a.batch / ms / 1e6:.3f} Gbit/s",
           flush=True)
+
+
+def timed(fn, steps, torch):
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(steps):
+        fn()
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps
+
+
+LIFTS = [16, 18, 20, 22, 24, 26, 28, 30, 32, 36, 40, 44, 48, 52, 56, 60, 64, 72, 80, 88, 96, 104, 112, 120, 128,
+         144, 160, 176, 192, 208, 224, 240, 256, 288, 320, 352, 384]
+
+
+def uneven(a, torch, F):
+    """The uneven workload: descriptors from a fixed seed, LLRs from the device generator."""
+    import numpy as np
+    rng = np.random.default_rng(7)
+    n = a.uneven
+    bgs = rng.integers(1, 3, n)
+    zs = np.array(LIFTS)[rng.integers(0, len(LIFTS), n)]
+    mods = np.where(rng.integers(0, 2, n) == 0, F.BPSK, F.QPSK)
+    limits = rng.integers(5, 26, n).astype(np.uint32)
+    codes = {}
+    for b, z in set(zip(bgs.tolist(), zs.tolist())):
+        codes[(b, z)] = F.NrLdpc.lifted(b, z).set_decoder("normalized_min_sum", 0.75, True, 25)
+    groups = {}
+    for i in range(n):
+        groups.setdefault((int(bgs[i]), int(zs[i]), int(mods[i])), []).append(i)
+    per = [None] * n
+    for gi, ((b, z, m), idx) in enumerate(sorted(groups.items())):
+        llr, _ = F.generate_training_batch(codes[(b, z)], len(idx), m, 0.0, 3.0, snr_kind=F.EBN0_DB, seed=7,
+                                           point=gi)
+        for j, i in enumerate(idx):
+            per[i] = llr[j]
+    cl = [codes[(int(b), int(z))] for b, z in zip(bgs, zs)]
+    offsets = np.zeros(n + 1, np.uint64)
+    offsets[1:] = np.cumsum([c.dims.e for c in cl])
+    flat = torch.cat(per).contiguous()
+    plan = F.BatchPlan(cl, offsets, limits, half=a.half)
+    out = plan.outputs()
+    if a.tune:
+        best, _ = plan.tune(flat, out, repeats=3)
+        print(f"tuned: {plan.dims.threads} lanes ({best:.3f} ms)", flush=True)
+    plan.run(flat, out=out, graph=a.graph)
+    ms = timed(lambda: plan.run(flat, out=out, graph=a.graph), a.steps, torch)
+    d = plan.dims
+    its = out[1].double()
+    print(f"uneven{' fp16' if a.half else ''}{' graph' if a.graph else ''}: {n} codewords, {d.classes} codes, "
+          f"{d.tasks} tasks, grid {d.grid} x {d.threads} lanes, LDS {d.lds_bytes} B, mean iterations "
+          f"{its.mean().item():.3f}, {ms:.3f} ms per launch, {n / ms * 1e3:.0f} codewords/s, "
+          f"{d.bits_total / ms / 1e6:.3f} Gbit/s", flush=True)
 
 
 if __name__ == "__main__":
