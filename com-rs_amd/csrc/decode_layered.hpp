@@ -308,6 +308,8 @@ __device__ __forceinline__ void sweep(typename Store<HALF>::T* post, int r, uint
             b = ab[2 * I + 1];
         }
         if constexpr (barrier_before<BG>(I)) __syncthreads();
+        // (the branch also bounds the scheduler's reach: without it the compiler interleaves
+        // layers and spills hundreds of registers)
         if (active) layer<BG, HALF, SOLO, ZC, I>(post, r, gofs, z, shift, msg, a, b);
         sweep<BG, HALF, SOLO, ZC, I + 1>(post, r, gofs, z, active, shift, msg, ab, a, b);
     }
@@ -318,6 +320,7 @@ struct Shared {
     uint32_t shift[NR_BG1_ENTRIES];
     uint16_t col[NR_BG1_ENTRIES];
     uint16_t row_start[48];
+    uint32_t synd[NR_BG1_ENTRIES];  // Z % 32 == 0: per edge, (word of its column, bit) in the packed decisions
     uint32_t fails[kMaxGroup];  // per codeword of the task: the syndrome stamp of a failing check
     uint32_t task;
     uint32_t vote[2];           // "some codeword still runs", alternate rounds
@@ -393,9 +396,9 @@ __device__ __forceinline__ bool syndrome_fails(const typename Store<HALF>::T* po
             const uint32_t* hd = hd_all + gg * per_cw + w;
             uint32_t par = 0;
             for (int eb = sh.row_start[i]; eb < sh.row_start[i + 1]; ++eb) {
-                const int v = int(sh.shift[eb]);
-                const int base = sh.col[eb] * 2 * w32 + (v >> 5);
-                par ^= __builtin_amdgcn_alignbit(hd[base + 1], hd[base], uint32_t(v & 31));
+                const uint32_t x = sh.synd[eb];
+                const uint32_t* q = hd + (x >> 5);
+                par ^= __builtin_amdgcn_alignbit(q[1], q[0], x & 31u);
             }
             if (par) sh.fails[gg] = stamp;
         }
@@ -586,9 +589,13 @@ __global__ void __launch_bounds__(kMaxThreads, HALF ? 3 : 1) decode_layered_kern
             // the shifts (read with uniform addresses), columns and row starts in LDS
             const int edges = c.bg == 1 ? Graph<1>::edges : Graph<2>::edges;
             const int rows = c.bg == 1 ? Graph<1>::rows : Graph<2>::rows;
+            const int w32 = c.z >> 5;
             for (int e = threadIdx.x; e < edges; e += blockDim.x) {
-                sh.shift[e] = c.shift32[e];
-                sh.col[e] = c.col[e];
+                const uint32_t v = c.shift32[e], j = c.col[e];
+                sh.shift[e] = v;
+                sh.col[e] = uint16_t(j);
+                // the funnel shift of the packed syndrome: word j 2 w32 + v / 32, bit v % 32
+                sh.synd[e] = ((j * 2u * uint32_t(w32) + (v >> 5)) << 5) | (v & 31u);
             }
             for (int i = threadIdx.x; i <= rows; i += blockDim.x) sh.row_start[i] = c.row_start[i];
             __syncthreads();
