@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 # the channel LLRs in and the decisions out, 453.6 MB per 4096 codewords
 TRAFFIC_PER_CODEWORD = 110745
 LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz
-LEARNED_ITERATIONS = 8
+LEARNED_ITERATIONS = 10
 K_INFO, E, Z, BG1_EDGES = 8448, 25344, 384, 316
 
 

@@ -119,3 +119,26 @@ def test_campaign_stops_on_frame_errors():
         m = c.metric("fer")
         assert c.status == "completed" and c.trials % 512 == 0 or c.trials == 20000
         assert m.count >= 50 or c.trials == 20000
+
+
+def test_campaign_with_the_learned_decoder_and_its_fallback():
+    """The learned link with the plain parameters (a = alpha, b = 0) and T = the code's limit
+    decodes every frame as the plain link; the fallback decodes the frames that did not
+    converge again, from their channel LLRs, with the same rule: the same decisions (equal
+    bit and frame error counts) and T more iterations on each of those frames."""
+    print("\n[campaign] learned link and fallback", flush=True)
+    code = F.NrLdpc.lifted(2, 52).set_decoder("normalized_min_sum", 0.75, True, 12)
+    grid = F.Grid().axis("ebn0_db", [1.0, 1.4])
+    stop = F.Stop.max_trials(4096)
+    plain = F.Runner().seed(9).batch(1024).stop(stop).run_nr(code, grid)
+    p = F.initial_params(code, 12)
+    same = F.Runner().seed(9).batch(1024).stop(stop).run_nr(code, grid, learned_params=p)
+    fb = F.Runner().seed(9).batch(1024).stop(stop).run_nr(code, grid, learned_params=p, fallback=True)
+    for a, b, c in zip(plain, same, fb):
+        assert a.trials == b.trials == c.trials == 4096
+        counts = [(r.metric("fer").count, r.metric("ber").count) for r in (a, b, c)]
+        assert counts[0] == counts[1] == counts[2]
+        assert b.metric("iterations").value == a.metric("iterations").value
+        extra = (c.metric("iterations").value - a.metric("iterations").value) * 4096 / 12
+        assert abs(extra - round(extra)) < 1e-6 * 4096  # whole frames of 12 more iterations
+        assert round(extra) >= a.metric("fer").count > 0
