@@ -41,6 +41,7 @@ SIGNATURES = [
     ("cf_last_error", C.c_char_p, []),
     ("cf_nr_create", C.c_int, [C.c_uint32, C.c_uint32, C.c_int, C.c_int, C.c_uint32, C.POINTER(C.c_void_p)]),
     ("cf_lifted_create", C.c_int, [C.c_int, C.c_uint32, C.POINTER(C.c_void_p)]),
+    ("cf_code_from_rows", C.c_int, [C.c_uint32, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_void_p)]),
     ("cf_code_destroy", None, [C.c_void_p]),
     ("cf_code_dims_get", C.c_int, [C.c_void_p, C.POINTER(Dims)]),
     ("cf_bit_map", C.c_int, [C.c_void_p, _u32p, C.c_size_t]),
@@ -166,6 +167,17 @@ class NrLdpc:
         """from_core_rows on the lifted base graph (ldpc.rs:270-287 on nr/ldpc.rs:124-133)."""
         h = C.c_void_p()
         _check(lib().cf_lifted_create(base_graph, z, C.byref(h)))
+        return cls(0, 0, _handle=h)
+
+    @classmethod
+    def from_rows(cls, n, rows):
+        """LdpcCode::from_rows (crates/com-fec/src/ldpc.rs:216-231): rows of H as lists of
+        variables; K = N - rank H at the non-pivot positions, sent whole (E = N)."""
+        start = np.zeros(len(rows) + 1, np.uint32)
+        start[1:] = np.cumsum([len(r) for r in rows])
+        cols = np.ascontiguousarray([v for r in rows for v in r], np.uint32)
+        h = C.c_void_p()
+        _check(lib().cf_code_from_rows(n, len(rows), start, cols, C.byref(h)))
         return cls(0, 0, _handle=h)
 
     def __del__(self):

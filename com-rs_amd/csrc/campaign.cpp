@@ -66,6 +66,7 @@ namespace detail {
 
 TrialGenArgs make_generator_args(const NrCode& c, int device, int modulation, int snr_kind, double snr_db,
                                  uint64_t seed, uint32_t point) {
+    if (c.bg == 0) fail(Kind::Config, "the device trial generator draws NR codewords; this code is given by its rows");
     if (modulation != CF_BPSK && modulation != CF_QPSK && modulation != CF_QAM16 && modulation != CF_QAM64)
         fail(Kind::Config, "the modulation must be BPSK, QPSK, 16-QAM or 64-QAM, got " + std::to_string(modulation));
     if (snr_kind != CF_EBN0_DB && snr_kind != CF_ESN0_DB)

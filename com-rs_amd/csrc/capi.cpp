@@ -62,6 +62,7 @@ DevCode dev_code(const NrCode& c, int device) {
     d.col_edge = t.col_edge;
     d.src_start = t.src_start;
     d.src = t.src;
+    d.info = t.info;
     return d;
 }
 
@@ -148,6 +149,13 @@ cf_status cf_lifted_create(int base_graph, uint32_t z, cf_code** out) {
     });
 }
 
+cf_status cf_code_from_rows(uint32_t n, uint32_t m, const uint32_t* row_start, const uint32_t* cols, cf_code** out) {
+    return guarded([&] {
+        if (!out) fail(Kind::Config, "null output handle");
+        *out = new cf_code(NrCode::from_rows(n, m, row_start, cols));
+    });
+}
+
 void cf_code_destroy(cf_code* code) { delete code; }
 
 cf_status cf_code_dims_get(const cf_code* h, cf_code_dims* out) {
@@ -211,7 +219,8 @@ cf_status cf_app_into(const cf_code* h, const double* llr, size_t e, const doubl
         expect_len(k_info, h->code.k_info, "a posteriori LLRs");
         std::vector<double> post(h->code.n);
         decode_host_f64(h, llr, false, apriori, post.data(), nullptr, iterations);
-        std::memcpy(out, post.data(), k_info * sizeof(double));
+        const std::vector<uint32_t>& info = h->code.info;
+        for (size_t i = 0; i < k_info; ++i) out[i] = post[info.empty() ? i : info[i]];
     });
 }
 

@@ -92,20 +92,21 @@ struct Block {
     T* chan;
     int tid, nthreads;
 
-    // whether some check fails on the hard decisions of post (ldpc.rs:555-557)
+    // whether some check fails on the hard decisions of post (ldpc.rs:555-557); the
+    // lanes take the (base row, lifted row) pairs, so Z = 1 codes check rows in parallel
     __device__ bool syndrome_fails() const {
         const DevCode& c = a.code;
         int bad = 0;
-        for (int r = tid; r < c.z; r += nthreads)
-            for (int i = 0; i < c.rows_b && !bad; ++i) {
-                int par = 0;
-                for (int eb = c.row_start[i]; eb < c.row_start[i + 1]; ++eb) {
-                    int rr = r + c.shift[eb];
-                    rr -= rr >= c.z ? c.z : 0;
-                    par ^= post[c.col[eb] * c.z + rr] < T(0);
-                }
-                bad |= par;
+        for (int task = tid; task < c.rows_b * c.z && !bad; task += nthreads) {
+            const int i = task / c.z, r = task - i * c.z;
+            int par = 0;
+            for (int eb = c.row_start[i]; eb < c.row_start[i + 1]; ++eb) {
+                int rr = r + c.shift[eb];
+                rr -= rr >= c.z ? c.z : 0;
+                par ^= post[c.col[eb] * c.z + rr] < T(0);
             }
+            bad |= par;
+        }
         return __syncthreads_or(bad) != 0;
     }
 
@@ -136,12 +137,14 @@ struct Block {
         }
     }
 
-    // ldpc.rs:533-544: new messages of every row from the old posteriors, in place
+    // ldpc.rs:533-544: new messages of every row from the old posteriors, in place; the
+    // rows are independent, so the lanes take (base row, lifted row) pairs
     __device__ void flood_checks(bool first) {
         const DevCode& c = a.code;
-        for (int i = 0; i < c.rows_b; ++i) {
+        for (int task = tid; task < c.rows_b * c.z; task += nthreads) {
+            const int i = task / c.z, r = task - i * c.z;
             const int e0 = c.row_start[i], deg = c.row_start[i + 1] - e0;
-            for (int r = tid; r < c.z; r += nthreads) {
+            {
                 T in[kMaxDeg], out[kMaxDeg];
 #pragma unroll
                 for (int q = 0; q < kMaxDeg; ++q)
@@ -203,10 +206,19 @@ __global__ void __launch_bounds__(512) decode_generic_kernel(GenericArgs<T> a) {
                 for (uint32_t s = c.src_start[p]; s < c.src_start[p + 1]; ++s) v += a.llr[w * c.e + c.src[s]];
                 if (p >= c.k_info && p < c.k) v = T(30);
             }
-            if (a.apriori && p < c.k_info) v += a.apriori[w * c.k_info + p];
             b.post[p] = v;
-            if (!a.layered) b.chan[p] = v;
         }
+        if (a.apriori) {
+            // a priori LLRs add to the channel LLRs of the information positions
+            __syncthreads();
+            for (int i = b.tid; i < c.k_info; i += b.nthreads) {
+                const int p = c.info ? int(c.info[i]) : i;
+                b.post[p] += a.apriori[w * c.k_info + i];
+            }
+        }
+        __syncthreads();
+        if (!a.layered)
+            for (int p = b.tid; p < c.n; p += b.nthreads) b.chan[p] = b.post[p];
         __syncthreads();
         int done = b.syndrome_fails() ? -1 : 0;
         for (int it = 1; it <= a.max_iterations && done < 0; ++it) {
@@ -226,7 +238,8 @@ __global__ void __launch_bounds__(512) decode_generic_kernel(GenericArgs<T> a) {
         if (done < 0) done = a.max_iterations;
         if (b.tid == 0 && a.iterations) a.iterations[w] = uint32_t(done);
         if (a.bits)
-            for (int p = b.tid; p < c.k_info; p += b.nthreads) a.bits[w * c.k_info + p] = b.post[p] < T(0);
+            for (int i = b.tid; i < c.k_info; i += b.nthreads)
+                a.bits[w * c.k_info + i] = b.post[c.info ? int(c.info[i]) : i] < T(0);
         if (a.post_out)
             for (int p = b.tid; p < c.n; p += b.nthreads) a.post_out[w * c.n + p] = b.post[p];
         __syncthreads();

@@ -20,6 +20,7 @@ struct DevCode {
     const uint16_t* col_edge;   // edges_b
     const uint32_t* src_start;  // n + 1
     const uint32_t* src;        // e
+    const uint32_t* info;       // k_info information positions, or null: 0 .. k_info - 1
 };
 
 template <typename T>

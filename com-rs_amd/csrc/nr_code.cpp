@@ -81,7 +81,8 @@ NrCode::NrCode(NrCode&& o) noexcept
       m(o.m), k_info(o.k_info), e(o.e), qm(o.qm), rv(o.rv), row_start(std::move(o.row_start)),
       col(std::move(o.col)), shift(std::move(o.shift)), col_start(std::move(o.col_start)),
       col_edge(std::move(o.col_edge)), map(std::move(o.map)), src_start(std::move(o.src_start)),
-      src(std::move(o.src)), core_inv(std::move(o.core_inv)), core_words(o.core_words), dec(o.dec),
+      src(std::move(o.src)), core_inv(std::move(o.core_inv)), core_words(o.core_words), info(std::move(o.info)),
+      sum_col(std::move(o.sum_col)), sum_start(std::move(o.sum_start)), sum_src(std::move(o.sum_src)), dec(o.dec),
       dev_(std::move(o.dev_)) {
     o.dev_.clear();
 }
@@ -126,6 +127,83 @@ NrCode NrCode::lifted(int bg, uint32_t z) {
     c.e = c.n - 2 * z;
     c.map.resize(c.e);
     for (uint32_t t2 = 0; t2 < c.e; ++t2) c.map[t2] = 2 * z + t2;
+    c.finish_tables();
+    return c;
+}
+
+NrCode NrCode::from_rows(uint32_t n, uint32_t m, const uint32_t* row_start, const uint32_t* cols) {
+    if (!row_start || (!cols && m > 0)) fail(Kind::Config, "null rows");
+    if (n < 1 || m < 1) fail(Kind::Config, std::to_string(m) + " checks for " + std::to_string(n) + " bits");
+    if (row_start[0] != 0) fail(Kind::Config, "the row starts begin at 0");
+    for (uint32_t r = 0; r < m; ++r)
+        if (row_start[r + 1] < row_start[r]) fail(Kind::Config, "the row starts must not decrease");
+    const uint32_t edges = row_start[m];
+    for (uint32_t e = 0; e < edges; ++e)
+        if (cols[e] >= n) fail(Kind::Config, "a check refers to a missing variable");
+    // the device tables hold 16-bit indices; the generic kernel takes rows of up to 19 edges
+    if (n > 65535 || edges > 65535 || m > 65535)
+        fail(Kind::Config, "codes given by rows are limited to 65535 bits, checks and edges on the device");
+    NrCode c;
+    c.bg = 0;
+    c.z = 1;
+    c.ils = 0;
+    c.rows_b = m;
+    c.cols_b = n;
+    c.edges_b = edges;
+    c.n = n;
+    c.m = m;
+    c.row_start.resize(m + 1);
+    for (uint32_t r = 0; r <= m; ++r) c.row_start[r] = uint16_t(row_start[r]);
+    c.col.resize(edges);
+    for (uint32_t e = 0; e < edges; ++e) c.col[e] = uint16_t(cols[e]);
+    c.shift.assign(edges, 0);
+    if (c.max_row_degree() > 19) fail(Kind::Config, "the device decoder takes checks of at most 19 variables");
+    c.col_start.assign(n + 1, 0);
+    for (uint32_t e = 0; e < edges; ++e) c.col_start[c.col[e] + 1]++;
+    for (uint32_t j = 0; j < n; ++j) c.col_start[j + 1] += c.col_start[j];
+    std::vector<uint16_t> fill(c.col_start.begin(), c.col_start.end() - 1);
+    c.col_edge.assign(edges, 0);
+    for (uint32_t e = 0; e < edges; ++e) c.col_edge[fill[c.col[e]]++] = uint16_t(e);
+    // gf2::Matrix::reduce (crates/com-math/src/gf2.rs:124-150): the reduced row echelon form
+    // with the pivots taken from the last column to the first (unique for that order)
+    const uint32_t words = (n + 63) / 64;
+    std::vector<uint64_t> h(size_t(m) * words, 0);
+    for (uint32_t r = 0; r < m; ++r)
+        for (uint32_t e = row_start[r]; e < row_start[r + 1]; ++e) h[size_t(r) * words + cols[e] / 64] ^= 1ull << (cols[e] % 64);
+    auto bit = [&](uint32_t r, uint32_t col) { return (h[size_t(r) * words + col / 64] >> (col % 64)) & 1; };
+    std::vector<uint32_t> pivots;
+    uint32_t rank = 0;
+    for (uint32_t colx = n; colx-- > 0 && rank < m;) {
+        uint32_t p = rank;
+        while (p < m && !bit(p, colx)) ++p;
+        if (p == m) continue;
+        if (p != rank)
+            std::swap_ranges(h.begin() + size_t(p) * words, h.begin() + size_t(p + 1) * words,
+                             h.begin() + size_t(rank) * words);
+        for (uint32_t r = 0; r < m; ++r)
+            if (r != rank && bit(r, colx))
+                for (uint32_t w = 0; w < words; ++w) h[size_t(r) * words + w] ^= h[size_t(rank) * words + w];
+        pivots.push_back(colx);
+        ++rank;
+    }
+    std::vector<uint8_t> is_pivot(n, 0);
+    for (uint32_t p : pivots) is_pivot[p] = 1;
+    for (uint32_t j = 0; j < n; ++j)
+        if (!is_pivot[j]) c.info.push_back(j);
+    if (c.info.empty()) fail(Kind::Config, "the code has no information bits");
+    c.sum_start.push_back(0);
+    for (uint32_t r = 0; r < rank; ++r) {
+        c.sum_col.push_back(pivots[r]);
+        for (uint32_t v : c.info)
+            if (bit(r, v)) c.sum_src.push_back(v);
+        c.sum_start.push_back(uint32_t(c.sum_src.size()));
+    }
+    c.k_info = uint32_t(c.info.size());
+    c.k = c.k_info;  // no fillers: the kernels fix positions [K', K) to 30.0
+
+    c.e = n;
+    c.map.resize(n);
+    for (uint32_t t = 0; t < n; ++t) c.map[t] = t;
     c.finish_tables();
     return c;
 }
@@ -237,10 +315,22 @@ void NrCode::encode_full(const uint8_t* u, uint8_t* cw) const {
         }
 }
 
-void NrCode::encode(const uint8_t* info, uint8_t* out) const {
-    std::vector<uint8_t> u(k, 0), cw(n);
-    std::memcpy(u.data(), info, k_info);
-    encode_full(u.data(), cw.data());
+void NrCode::encode(const uint8_t* bits, uint8_t* out) const {
+    std::vector<uint8_t> cw(n, 0);
+    if (bg == 0) {
+        // Encoder::Sums (ldpc.rs:760-764): the information bits in place, every pivot bit
+        // the sum of the information bits of its reduced row
+        for (uint32_t i = 0; i < k_info; ++i) cw[info[i]] = bits[i] & 1;
+        for (size_t r = 0; r + 1 < sum_start.size(); ++r) {
+            uint8_t a = 0;
+            for (uint32_t s = sum_start[r]; s < sum_start[r + 1]; ++s) a ^= cw[sum_src[s]];
+            cw[sum_col[r]] = a;
+        }
+    } else {
+        std::vector<uint8_t> u(k, 0);
+        std::memcpy(u.data(), bits, k_info);
+        encode_full(u.data(), cw.data());
+    }
     for (uint32_t t = 0; t < e; ++t) out[t] = cw[map[t]];
 }
 
@@ -252,7 +342,7 @@ const NrCode::DeviceTables& NrCode::device_tables(int device) const {
     // one buffer: u16 tables, then u32 tables (4-byte aligned)
     size_t n16 = row_start.size() + col.size() + shift.size() + col_start.size() + col_edge.size();
     n16 = (n16 + 1) & ~size_t(1);
-    size_t n32 = src_start.size() + src.size() + map.size() + shift.size();
+    size_t n32 = src_start.size() + src.size() + map.size() + shift.size() + info.size();
     n32 = (n32 + 1) & ~size_t(1);
     size_t bytes = n16 * 2 + n32 * 4 + core_inv.size() * 8;
     std::vector<uint8_t> host(bytes, 0);
@@ -272,6 +362,7 @@ const NrCode::DeviceTables& NrCode::device_tables(int device) const {
     std::memcpy(h32 + src_start.size() + src.size(), map.data(), map.size() * 4);
     uint32_t* h_shift = h32 + src_start.size() + src.size() + map.size();
     for (size_t i = 0; i < shift.size(); ++i) h_shift[i] = shift[i];
+    std::memcpy(h_shift + shift.size(), info.data(), info.size() * 4);
     uint64_t* h64 = reinterpret_cast<uint64_t*>(host.data() + n16 * 2 + n32 * 4);
     std::memcpy(h64, core_inv.data(), core_inv.size() * 8);
     int prev = 0;
@@ -297,6 +388,7 @@ const NrCode::DeviceTables& NrCode::device_tables(int device) const {
     t.src = d32 + src_start.size();
     t.map = d32 + src_start.size() + src.size();
     t.shift32 = d32 + src_start.size() + src.size() + map.size();
+    t.info = info.empty() ? nullptr : t.shift32 + shift.size();
     t.core_inv = reinterpret_cast<const uint64_t*>(static_cast<const uint8_t*>(base) + n16 * 2 + n32 * 4);
     return t;
 }

@@ -55,10 +55,17 @@ struct NrCode {
     // encoder: row j of the inverse of the core block, as a bit mask over the 4Z checks
     std::vector<uint64_t> core_inv;   // 4Z rows x words
     uint32_t core_words = 0;
+    // a code given by its rows (bg 0): the information positions (the columns that are not
+    // pivots of H's reduced form) and, per pivot column, the information bits it sums
+    std::vector<uint32_t> info;
+    std::vector<uint32_t> sum_col, sum_start, sum_src;
 
     DecoderSettings dec;
 
     static NrCode lifted(int bg, uint32_t z);
+    // LdpcCode::from_rows (crates/com-fec/src/ldpc.rs:216-231): m checks listing their
+    // variables, row r = cols[row_start[r] .. row_start[r+1]); sent whole (E = N)
+    static NrCode from_rows(uint32_t n, uint32_t m, const uint32_t* row_start, const uint32_t* cols);
     static NrCode with_options(uint32_t k_info, uint32_t e, int bg, int rv, uint32_t qm);
 
     uint32_t max_row_degree() const;
@@ -82,6 +89,7 @@ struct NrCode {
         const uint32_t* map = nullptr;
         const uint64_t* core_inv = nullptr;
         const uint32_t* shift32 = nullptr;  // the shifts as 32-bit words (scalar loads)
+        const uint32_t* info = nullptr;     // K' information positions (codes given by rows), or null: 0..K'-1
     };
     const DeviceTables& device_tables(int device) const;
     ~NrCode();

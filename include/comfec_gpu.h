@@ -73,6 +73,15 @@ cf_status cf_nr_create(uint32_t k_info, uint32_t e, int base_graph, int rv, uint
    transmitted once, in order. */
 cf_status cf_lifted_create(int base_graph, uint32_t z, cf_code** out);
 
+/* LdpcCode::from_rows (crates/com-fec/src/ldpc.rs:216-231): the code of the parity-check
+   matrix whose row r lists the variables cols[row_start[r] .. row_start[r+1]) (m + 1 row
+   starts); K = N - rank H, the information positions are the columns that are not pivots of
+   H's reduced form (pivots taken from the last column, gf2.rs:124-150), the encoder sums
+   them (ldpc.rs:760-764), and the block is sent whole (E = N). Decoded by the general
+   kernel (any rule, layered or flooding); rows of at most 19 variables, N, M and the edges
+   at most 65535. cf_decode_block and cf_app_into return the information positions. */
+cf_status cf_code_from_rows(uint32_t n, uint32_t m, const uint32_t* row_start_host,
+                            const uint32_t* cols_host, cf_code** out);
 void cf_code_destroy(cf_code* code);
 
 /* NrLdpc::{k, n, base_graph, lifting_size} and LdpcCode::{n, k} (nr/ldpc.rs:164-187). */

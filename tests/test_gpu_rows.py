@@ -1,0 +1,119 @@
+"""Codes given by their parity-check rows on the GPU (the general kernel), through the C ABI:
+the AFF3CT fixture configurations of validation/aff3ct/*.json (PEG matrices, BPSK over AWGN,
+the all-zero codeword AFF3CT sends) replayed on the device.
+
+- f64 messages: posteriors and iteration counts bit-identical to the oracle (layered NMS,
+  flooding OMS; sum-product within the device's tanh/atanh rounding);
+- FP32 messages: bit-identical to the FP32 oracle;
+- frame error rates of the device decoder inside the 99 % Wilson interval of AFF3CT's
+  (the acceptance rule of validation/tests/aff3ct.rs:296-333);
+- the host entry points return the information positions (decode_block, app_into).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import comfec as F
+import oracle as O
+from alist import read_alist
+from conftest import GOLDEN
+from gf2 import nonpivot_columns
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+AFF3CT = [
+    ("ldpc_peg_n576_k288_layered_nms075_i20", "peg_n576_k288.alist", O.NMS, "normalized_min_sum", 0.75, True, 20),
+    ("ldpc_peg_n576_k288_flooding_oms015_i20", "peg_n576_k288.alist", O.OMS, "offset_min_sum", 0.15, False, 20),
+    ("ldpc_peg_n576_k288_flooding_spa_i50", "peg_n576_k288.alist", O.SPA, "sum_product", 0.0, False, 50),
+    ("ldpc_peg_n648_k540_layered_spa_i20", "peg_n648_k540.alist", O.SPA, "sum_product", 0.0, True, 20),
+]
+
+
+def zero_codeword_llrs(name, n, ebn0_db, rate, frames, cond=0):
+    """The oracle's BPSK/AWGN chain on the all-zero codeword, stream (name, cond, t)."""
+    s2 = O.lib.or_bpsk_noise_variance(ebn0_db, rate)
+    seed = O.fnv1a64(name.encode())
+    llr = np.zeros((frames, n))
+    zero = np.zeros(n, np.uint8)
+    for t in range(frames):
+        r = O.Rng.stream(seed, name, cond, t)
+        O.lib.or_bpsk_awgn_llr(zero, n, s2, r.r, llr[t])
+    return llr
+
+
+def setup(matrix, name, alpha, layered, iters):
+    n, rows = read_alist(os.path.join(GOLDEN, matrix))
+    code = F.NrLdpc.from_rows(n, rows).set_decoder(name, alpha, layered, iters)
+    return n, rows, code, O.Code.from_rows(n, rows)
+
+
+@pytest.mark.parametrize("cfg", AFF3CT, ids=[a[0] for a in AFF3CT])
+def test_device_matches_the_oracle(cfg):
+    fx_name, matrix, orule, name, alpha, layered, iters = cfg
+    print(f"\n[rows] {fx_name}: device vs oracle", flush=True)
+    n, rows, code, ref = setup(matrix, name, alpha, layered, iters)
+    fx = json.load(open(os.path.join(GOLDEN, "aff3ct_" + fx_name + ".json")))
+    k = int(fx["parameters"]["Codec"]["Info. bits (K)"])
+    ebn0 = fx["data"][len(fx["data"]) // 2]["ebn0_db"]
+    llr = zero_codeword_llrs(fx_name, n, ebn0, k / n, 128, cond=7)
+    post_ref, it_ref = ref.decode_many(llr, rule=orule, alpha=alpha, layered=layered, max_iterations=iters, threads=16)
+    bits, its, post = code.decode_batch(torch.from_numpy(llr).cuda(), want_post=True)
+    post = post.cpu().numpy()
+    assert np.array_equal(its.cpu().numpy(), it_ref)
+    if orule == O.SPA:
+        # device tanh/atanh against glibc's: decisions equal, posteriors within 1e-9 relative
+        assert np.array_equal(post < 0, post_ref < 0)
+        assert np.max(np.abs(post - post_ref) / np.maximum(np.abs(post_ref), 1.0)) < 1e-9
+    else:
+        assert np.array_equal(post, post_ref)
+        p32, i32 = ref.decode_many_f32(llr.astype(np.float32), rule=orule, alpha=alpha, layered=layered,
+                                       max_iterations=iters, threads=16)
+        _, its32, post32 = code.decode_batch(torch.from_numpy(llr.astype(np.float32)).cuda(), want_post=True)
+        assert np.array_equal(its32.cpu().numpy(), i32)
+        assert np.array_equal(post32.cpu().numpy(), p32)
+    assert len(set(it_ref.tolist())) >= 3
+
+
+@pytest.mark.parametrize("cfg", AFF3CT, ids=[a[0] for a in AFF3CT])
+def test_device_fer_inside_aff3ct_intervals(cfg):
+    fx_name, matrix, orule, name, alpha, layered, iters = cfg
+    print(f"\n[rows] {fx_name}: FER vs AFF3CT", flush=True)
+    n, rows, code, ref = setup(matrix, name, alpha, layered, iters)
+    fx = json.load(open(os.path.join(GOLDEN, "aff3ct_" + fx_name + ".json")))
+    k = int(fx["parameters"]["Codec"]["Info. bits (K)"])
+    checked = 0
+    for ci, p in enumerate(fx["data"]):
+        if p["fer"] < 2e-3 or p["fer"] > 0.9:
+            continue
+        frames = int(min(20000, max(2000, 200 / p["fer"])))
+        llr = zero_codeword_llrs(fx_name, n, p["ebn0_db"], k / n, frames, cond=ci)
+        _, _, post = code.decode_batch(torch.from_numpy(llr.astype(np.float32)).cuda(), want_post=True)
+        fe = int((post < 0).any(1).sum())
+        lo, hi = O.wilson(fe, frames, 0.99)
+        alo, ahi = O.wilson(p["frame_errors"], p["frames"], 0.99)
+        print(f"  {p['ebn0_db']} dB: device {fe}/{frames} = {fe / frames:.4g}, AFF3CT {p['fer']:.4g}", flush=True)
+        assert not (ahi < lo or hi < alo), (fx_name, p["ebn0_db"], fe / frames, p["fer"])
+        checked += 1
+    assert checked >= 2
+
+
+def test_host_entry_points_return_the_information_positions():
+    print("\n[rows] decode_block / app_into", flush=True)
+    n, rows, code, ref = setup("peg_n576_k288.alist", "normalized_min_sum", 0.75, True, 20)
+    info = np.array(nonpivot_columns(n, rows))
+    rng = np.random.default_rng(4)
+    u = rng.integers(0, 2, code.k).astype(np.uint8)
+    cw = code.encode_block(u)
+    s2 = O.lib.or_bpsk_noise_variance(2.5, code.k / n)
+    llr = 2.0 * ((1.0 - 2.0 * cw) + rng.normal(0.0, np.sqrt(s2), n)) / s2
+    post_ref, it_ref = ref.decode_llr(llr, rule=O.NMS, alpha=0.75, layered=True, max_iterations=20)
+    assert np.array_equal(code.decode_block(llr), (post_ref[info] < 0).astype(np.uint8))
+    apriori = rng.normal(0.0, 2.0, code.k)
+    shifted = llr.copy()
+    shifted[info] += apriori
+    post2, it2 = ref.decode_llr(shifted, rule=O.NMS, alpha=0.75, layered=True, max_iterations=20)
+    out, it = code.app(llr, apriori)
+    assert it == it2 and np.array_equal(out, post2[info])
