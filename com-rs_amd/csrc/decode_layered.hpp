@@ -140,17 +140,19 @@ template <>
 struct Store<true> {
     using T = uint16_t;  // binary16 bits
     static __device__ __forceinline__ float ld(const uint16_t* p, int i) { return h2f(p[i]); }
-    static __device__ __forceinline__ void st(uint16_t* p, int i, float x) {
-        const uint32_t h = f2h(__builtin_amdgcn_fmed3f(x, -65504.0f, 65504.0f));
-        p[i] = uint16_t((h & 0x7fffu) ? h : 0u);  // a zero is stored as +0.0
+    // half(clamp(x)), a zero stored as +0.0 (the half sum -0 + +0 is +0, and x + 0 is x
+    // for every other half)
+    static __device__ __forceinline__ uint16_t to_store(float x) {
+        const _Float16 h = _Float16(__builtin_amdgcn_fmed3f(x, -65504.0f, 65504.0f));
+        return __builtin_bit_cast(uint16_t, _Float16(h + _Float16(0.0f)));
     }
+    static __device__ __forceinline__ void st(uint16_t* p, int i, float x) { p[i] = to_store(x); }
     static __device__ __forceinline__ uint32_t sign(const uint16_t* p, int i) { return uint32_t(p[i]) >> 15; }
     static __device__ __forceinline__ float ld_at(uint32_t off) {
         return h2f(*(const lds_ptr<uint16_t>::type)(uintptr_t)off);
     }
     static __device__ __forceinline__ void st_at(uint32_t off, float x) {
-        const uint32_t h = f2h(__builtin_amdgcn_fmed3f(x, -65504.0f, 65504.0f));
-        *(lds_ptr<uint16_t>::type)(uintptr_t)off = uint16_t((h & 0x7fffu) ? h : 0u);
+        *(lds_ptr<uint16_t>::type)(uintptr_t)off = to_store(x);
     }
 };
 
@@ -250,29 +252,37 @@ __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post
     msg.template get<I>(g1o, g2o);
     const uint32_t pko = msg.pk[I];
     const uint32_t i1o = pko >> 24;
-    float v2c[deg];
-    int addr[deg];  // element index; with ZC, the LDS byte address (the posteriors start at 0)
-    float m1 = INFINITY, m2 = INFINITY;
-    uint32_t parity = 0;
     constexpr uint32_t es = sizeof(typename S::T);
-#pragma unroll
-    for (int q = 0; q < deg; ++q) {
-        // (r + shift) mod z: the unsigned min of t and t - z (which wraps when t < z)
+    // (r + shift) mod z for edge q: the unsigned min of t and t - z (which wraps when t < z);
+    // with ZC, the LDS byte address (the posteriors start at 0) so that the column base is
+    // the LDS instruction's immediate offset, and a zero shift (every extension column)
+    // needs no wrap; otherwise the element index
+    auto address = [&](int q) -> int {
         if constexpr (ZC != 0) {
-            // in bytes, so that the column base is the LDS instruction's immediate offset; a
-            // zero shift (every extension column) needs no wrap
             const uint32_t s_q = shift_of<BG, ZC>(e0 + q) * es;
             const uint32_t t = uint32_t(r) * es + s_q;
             const uint32_t rr = s_q == 0 ? t : min(t, t - uint32_t(ZC) * es);
-            addr[q] = int(rr) + int(Graph<BG>::t[e0 + q].j) * ZC * int(es);
+            return int(rr) + int(Graph<BG>::t[e0 + q].j) * ZC * int(es);
         } else {
             const uint32_t t = uint32_t(r) + shift[e0 + q];
             const uint32_t rr = min(t, t - uint32_t(z));
-            addr[q] = int(Graph<BG>::t[e0 + q].j) * z + int(rr) + int(base);
+            return int(Graph<BG>::t[e0 + q].j) * z + int(rr) + int(base);
         }
+    };
+    // FP16 kernels run three waves a SIMD (168 registers): their widest layers compute the
+    // addresses twice rather than hold them
+    constexpr bool keep = !HALF || deg <= 10;
+    float v2c[deg];
+    int addr[keep ? deg : 1];
+    float m1 = INFINITY, m2 = INFINITY;
+    uint32_t parity = 0;
+#pragma unroll
+    for (int q = 0; q < deg; ++q) {
+        const int ad = address(q);
+        if constexpr (keep) addr[q] = ad;
         const float mag = uint32_t(q) == i1o ? g2o : g1o;
         const float old = __uint_as_float(__float_as_uint(mag) ^ ((pko << (32 - deg + q)) & 0x80000000u));
-        const float in = (ZC != 0 ? S::ld_at(uint32_t(addr[q])) : S::ld(post, addr[q])) - old;
+        const float in = (ZC != 0 ? S::ld_at(uint32_t(ad)) : S::ld(post, ad)) - old;
         v2c[q] = in;
         parity ^= __float_as_uint(in);
         const float m = fabsf(in);
@@ -292,8 +302,9 @@ __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post
         const uint32_t sb = __float_as_uint(v2c[q]) ^ parity;
         signs = __builtin_amdgcn_alignbit(signs, sb, 31u);
         const float out = __uint_as_float(__float_as_uint(mag) ^ (sb & 0x80000000u));
-        if constexpr (ZC != 0) S::st_at(uint32_t(addr[q]), v2c[q] + out);
-        else S::st(post, addr[q], v2c[q] + out);
+        const int ad = keep ? addr[keep ? q : 0] : address(q);
+        if constexpr (ZC != 0) S::st_at(uint32_t(ad), v2c[q] + out);
+        else S::st(post, ad, v2c[q] + out);
     }
     msg.pk[I] = signs | (i1 << 24);
 }
