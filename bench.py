@@ -47,26 +47,59 @@ def parse():
 
 
 def cpu_baseline(llr_f32, seconds):
-    """The oracle's f64 layered normalized min-sum (a C restatement of LdpcCode::propagate,
-    10 iterations, syndrome stop) on host threads over a bounded sample of the timed batch."""
+    """The oracle's f64 layered NMS (a C restatement of LdpcCode::propagate, 10 iterations,
+    syndrome stop) on host threads over a bounded sample of the timed batch, on all cores
+    and on one; and, once, the one published timing of the reference's decoder
+    (benchmarks/libraries/results/s4_coding.json:248,333: NR K'=1024 E=2048, BG2 Z=104,
+    flooding sum-product f64, at most 50 iterations, 200 frames at Eb/N0 2.0 dB, 0.34056 s
+    on an Apple M6 thread) for the ratio."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     ref = O.Code.nr(K_INFO, E, 1, 0, 4)
     threads = max(1, min(16, os.cpu_count() or 1))
+
+    def rate(count, nthreads):
+        fulls = np.array([ref.full_llr(l.astype(np.float64)) for l in llr_f32[:count]])
+        t0 = time.perf_counter()
+        ref.decode_many(fulls, rule=O.NMS, alpha=0.75, layered=True, max_iterations=10, threads=nthreads)
+        return time.perf_counter() - t0
+
     probe = min(threads, llr_f32.shape[0])
-    fulls = np.array([ref.full_llr(l.astype(np.float64)) for l in llr_f32[:probe]])
-    t0 = time.perf_counter()
-    ref.decode_many(fulls, rule=O.NMS, alpha=0.75, layered=True, max_iterations=10, threads=threads)
-    dt = time.perf_counter() - t0
+    dt = rate(probe, threads)
     count = int(max(probe, min(llr_f32.shape[0], probe * seconds / max(dt, 1e-3))))
-    fulls = np.array([ref.full_llr(l.astype(np.float64)) for l in llr_f32[:count]])
-    t0 = time.perf_counter()
-    ref.decode_many(fulls, rule=O.NMS, alpha=0.75, layered=True, max_iterations=10, threads=threads)
-    dt = time.perf_counter() - t0
+    dt = rate(count, threads)
+    one = max(1, min(count, int(2.0 * count / max(dt * threads, 1e-3))))  # about 2 s on one thread
+    dt1 = rate(one, 1)
+
+    # the published workload, same chain as the reference's benchmark: random bits, the NR
+    # encoder, BPSK over AWGN at the rate K'/E, f64 LLRs 2y/sigma^2
+    cal = O.Code.nr(1024, 2048)
+    s2 = O.lib.or_bpsk_noise_variance(2.0, 1024 / 2048)
+    fulls = []
+    for t in range(200):
+        r = O.Rng.stream(1, "s4_coding", 0, t)
+        info = np.array([r.bit() for _ in range(1024)], np.uint8)
+        llr = np.zeros(2048)
+        O.lib.or_bpsk_awgn_llr(cal.nr_encode(info), 2048, s2, r.r, llr)
+        fulls.append(cal.full_llr(llr))
+    fulls = np.array(fulls)
+    cal.decode_many(fulls, rule=O.SPA, alpha=0.0, layered=False, max_iterations=50, threads=1)
+    runs = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        cal.decode_many(fulls, rule=O.SPA, alpha=0.0, layered=False, max_iterations=50, threads=1)
+        runs.append(time.perf_counter() - t0)
+    cal_s = float(np.median(runs))
     return {"value": round(K_INFO * count / dt / 1e9, 6), "unit": "Gbit/s", "cores": threads, "kind": "port",
             "sample": f"{count} codewords of the timed batch: f64 layered NMS 0.75, 10 it, syndrome stop "
-                      f"(oracle C restatement of LdpcCode::propagate), {threads} threads, {dt:.1f} s"}
+                      f"(oracle C restatement of LdpcCode::propagate), {threads} threads, {dt:.1f} s",
+            "single_thread": {"value": round(K_INFO * one / dt1 / 1e9, 6), "unit": "Gbit/s",
+                              "sample": f"{one} codewords, 1 thread, {dt1:.2f} s"},
+            "calibration": {"workload": "NR K'=1024 E=2048 (BG2 Z=104), flooding sum-product f64, <= 50 it, "
+                                        "200 frames at Eb/N0 2.0 dB, 1 thread, median of 5",
+                            "seconds": round(cal_s, 4), "reference_seconds_apple_m6": 0.34056,
+                            "ratio_to_reference": round(cal_s / 0.34056, 3)}}
 
 
 def frame_errors(bits, info, F):

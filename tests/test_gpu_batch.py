@@ -267,3 +267,33 @@ def test_plan_errors():
     plan = F.BatchPlan([a, b], np.array([0, a.dims.e, a.dims.e + b.dims.e], np.uint64))
     with pytest.raises(F.ComfecError):
         plan.set_threads(96)
+
+
+@pytest.mark.parametrize("k,e,rv,qm", [(500, 1200, 2, 4), (60, 1000, 1, 2), (1000, 1500, 3, 6),
+                                       (8448, 12672, 2, 6)])
+def test_rate_matching_variants_on_the_fast_kernel(k, e, rv, qm):
+    """The layered kernel's LLR placement on NR codes with fillers, repetitions (E > Ncb),
+    redundancy versions 1-3 and Qm interleaving (nr/ldpc.rs:136-197), LLRs of the device
+    generator: FP32 and FP16 storage bit-identical to their oracles, whose full LLRs are
+    summed in FP32 in output order as the kernel sums them."""
+    print(f"\n[rate matching] k={k} e={e} rv={rv} qm={qm}", flush=True)
+    code = F.NrLdpc(k, e, None, rv, qm).set_decoder("normalized_min_sum", 0.75, True, 12)
+    ref = O.Code.nr(k, e, 0, rv, qm)
+    mod = {1: F.BPSK, 2: F.QPSK, 4: F.QAM16, 6: F.QAM64}[qm]
+    llr, _ = F.generate_batch(code, 48, mod, 2.0 + qm, snr_kind=F.EBN0_DB, seed=12)
+    host = llr.cpu().numpy()
+    pos = ref.bit_map()
+    full = np.zeros((48, ref.n), np.float32)
+    for t in range(e):
+        full[:, pos[t]] += host[:, t]  # float32 sums in output order
+    full[:, ref.k_info:ref.k] = 30.0
+    for half in (False, True):
+        bits, its, post = code.decode_batch(llr, want_post=True, half=half)
+        if half:
+            pr, ir = ref.decode_many_f16(full, rule=O.NMS, alpha=0.75, max_iterations=12, threads=THREADS)
+        else:
+            pr, ir = ref.decode_many_f32(full, rule=O.NMS, alpha=0.75, layered=True, max_iterations=12,
+                                         threads=THREADS)
+        assert np.array_equal(its.cpu().numpy(), ir), half
+        assert np.array_equal(post.cpu().numpy().view(np.uint32), pr.view(np.uint32)), half
+        assert np.array_equal(bits.cpu().numpy(), (pr[:, :k] < 0).astype(np.uint8)), half
