@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libcomfec_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["nr_code.cpp", "capi.cpp", "sim.cpp", "campaign.cpp", "learned_api.cpp", "batch.cpp", "decode_generic.hip", "decode_layered.hip", "decode_layered_f32.hip", "decode_layered_f16.hip", "trials.hip", "learned.hip"]
+SOURCES = ["nr_code.cpp", "capi.cpp", "sim.cpp", "sim_report.cpp", "campaign.cpp", "hooks_api.cpp", "learned_api.cpp", "batch.cpp", "decode_generic.hip", "decode_layered.hip", "decode_layered_f32.hip", "decode_layered_f16.hip", "trials.hip", "learned.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -22,9 +22,25 @@ FLAGS = [
 ]
 
 
-def _compile(src):
+def commit():
+    """The commit the library is built from, as crates/com-sim/build.rs records it: the hash,
+    with "-dirty" when the library's tracked sources have uncommitted changes, or "unknown"
+    outside a Git checkout."""
+    def git(*args):
+        r = subprocess.run(["git", "-C", HERE, *args], capture_output=True, text=True)
+        return r.stdout.strip() if r.returncode == 0 else None
+    top = git("rev-parse", "--show-toplevel")
+    head = top and git("rev-parse", "HEAD")
+    if not head:
+        return "unknown"
+    status = git("--no-optional-locks", "status", "--porcelain", "--untracked-files=no", "--",
+                 os.path.relpath(HERE, top), "include")
+    return head + "-dirty" if status else head
+
+
+def _compile(src, defines=()):
     obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *FLAGS, *defines, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
@@ -38,8 +54,9 @@ def _compile(src):
 
 def build(verbose=True):
     os.makedirs(OUT_DIR, exist_ok=True)
+    defines = {"sim_report.cpp": ['-DCOMFEC_COMMIT="%s"' % commit()]}
     with ThreadPoolExecutor(max_workers=4) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda s: _compile(s, defines.get(s, ())), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"]
     subprocess.run(cmd, check=True)

@@ -65,7 +65,39 @@ class GridAxis(C.Structure):
 
 
 class RunnerConfig(C.Structure):
-    _fields_ = [("seed", C.c_uint64), ("batch", C.c_uint64), ("confidence", C.c_double), ("fast_dev_run", C.c_int)]
+    _fields_ = [("seed", C.c_uint64), ("batch", C.c_uint64), ("confidence", C.c_double), ("fast_dev_run", C.c_int),
+                ("threads", C.c_uint64), ("config_json", C.c_char_p), ("hooks", C.c_void_p)]
+
+
+class RunInfoC(C.Structure):
+    _fields_ = [("scenario", C.c_char_p), ("seed", C.c_uint64), ("batch", C.c_uint64), ("threads", C.c_uint64),
+                ("stop", C.c_char_p), ("confidence", C.c_double), ("n_conditions", C.c_uint64),
+                ("started_unix_s", C.c_uint64), ("version", C.c_char_p), ("commit", C.c_char_p),
+                ("config", C.c_char_p)]
+
+
+class ProgressC(C.Structure):
+    _fields_ = [("condition", C.c_uint64), ("trials", C.c_uint64), ("elapsed_s", C.c_double),
+                ("record", C.c_void_p)]
+
+
+ON_RUN_START = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(RunInfoC))
+ON_CONDITION_START = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_double), C.c_size_t)
+ON_BATCH_END = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(ProgressC))
+ON_CONDITION_END = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p)
+ON_RUN_END = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p)
+LOG_START = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(RunInfoC))
+LOG_LOG = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_double)
+LOG_FINISH = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(RunInfoC), C.c_void_p, C.c_double)
+
+
+class CallbackC(C.Structure):
+    _fields_ = [("user", C.c_void_p), ("on_run_start", ON_RUN_START), ("on_condition_start", ON_CONDITION_START),
+                ("on_batch_end", ON_BATCH_END), ("on_condition_end", ON_CONDITION_END), ("on_run_end", ON_RUN_END)]
+
+
+class LoggerC(C.Structure):
+    _fields_ = [("user", C.c_void_p), ("start", LOG_START), ("log", LOG_LOG), ("finish", LOG_FINISH)]
 
 
 class NrLink(C.Structure):
@@ -96,6 +128,19 @@ SIGNATURES += [
                                        C.POINTER(C.c_double)]),
     ("cf_results_metric", C.c_int, [C.c_void_p, C.c_size_t, C.c_char_p, C.c_double, C.POINTER(Metric)]),
     ("cf_results_free", None, [C.c_void_p]),
+    ("cf_results_values", C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t), C.POINTER(C.c_double),
+                                    C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("cf_results_metric_count", C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("cf_results_metric_name", C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.POINTER(C.c_char_p)]),
+    ("cf_record_metric", C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.POINTER(Metric)]),
+    ("cf_hooks_create", C.c_int, [C.POINTER(C.c_void_p)]),
+    ("cf_hooks_add_callback", C.c_int, [C.c_void_p, C.POINTER(CallbackC)]),
+    ("cf_hooks_add_logger", C.c_int, [C.c_void_p, C.POINTER(LoggerC)]),
+    ("cf_hooks_progress_bar", C.c_int, [C.c_void_p, C.c_double]),
+    ("cf_hooks_skip_after", C.c_int, [C.c_void_p, C.c_char_p, C.c_double, C.c_char_p]),
+    ("cf_hooks_csv_logger", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("cf_hooks_json_logger", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("cf_hooks_destroy", None, [C.c_void_p]),
     ("cf_wilson", C.c_int, [C.c_uint64, C.c_uint64, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
 ]
 
@@ -361,7 +406,39 @@ class Record:
 STATUS = {0: "completed", 1: "stopped", 2: "skipped", 3: "dev_run"}
 
 
+class Action:
+    """What a callback asks the runner to do (callbacks.rs:50-61)."""
+    CONTINUE, SKIP, STOP_CONDITION, STOP_RUN = 0, 1, 2, 3
+
+
+class RunInfo:
+    """What the callbacks and loggers know about the run (runner.rs:150-175)."""
+
+    def __init__(self, c):
+        d = lambda b: b.decode() if b is not None else None
+        self.scenario, self.seed, self.batch, self.threads = d(c.scenario), c.seed, c.batch, c.threads
+        self.stop, self.confidence, self.n_conditions = d(c.stop), c.confidence, c.n_conditions
+        self.started_unix_s, self.version, self.commit, self.config = (c.started_unix_s, d(c.version), d(c.commit),
+                                                                       d(c.config))
+
+
+class Progress:
+    """The state of a condition after a batch (stopping.rs:48-59); `metric` reads its merged
+    record, valid during the callback only."""
+
+    def __init__(self, c):
+        self.condition, self.trials, self.elapsed_s, self._rec = c.condition, c.trials, c.elapsed_s, c.record
+
+    def metric(self, name, confidence=0.95):
+        m = Metric()
+        if lib().cf_record_metric(self._rec, name.encode(), confidence, C.byref(m)) != CF_OK:
+            return None
+        return m
+
+
 class ConditionResult:
+    """runner.rs:80-104: a condition's status, trials, wall time and metrics."""
+
     def __init__(self, index, condition, status, trials, wall_s, metrics, confidence):
         self.index, self.condition, self.status, self.trials, self.wall_s = index, condition, status, trials, wall_s
         self.metrics, self.confidence = metrics, confidence
@@ -370,22 +447,30 @@ class ConditionResult:
         return self.metrics.get(name)
 
 
-def _collect(res, conditions, confidence, names):
-    out = []
+def _result(res, i, keys, confidence):
+    st, tr, wall = C.c_int(), C.c_uint64(), C.c_double()
+    _check(lib().cf_results_condition(res, i, C.byref(st), C.byref(tr), C.byref(wall)))
+    index, n = C.c_size_t(), C.c_size_t()
+    vals = (C.c_double * max(1, len(keys)))()
+    _check(lib().cf_results_values(res, i, C.byref(index), vals, len(keys), C.byref(n)))
+    count = C.c_size_t()
+    _check(lib().cf_results_metric_count(res, i, C.byref(count)))
+    metrics = {}
+    for j in range(count.value):
+        name = C.c_char_p()
+        _check(lib().cf_results_metric_name(res, i, j, C.byref(name)))
+        m = Metric()
+        _check(lib().cf_results_metric(res, i, name.value, confidence, C.byref(m)))
+        metrics[name.value.decode()] = m
+    cond = {k: vals[q] for q, k in enumerate(keys[:n.value])}
+    return ConditionResult(index.value, cond, STATUS[st.value], tr.value, wall.value, metrics, confidence)
+
+
+def _collect(res, keys, confidence):
     try:
-        for i in range(lib().cf_results_count(res)):
-            st, tr, wall = C.c_int(), C.c_uint64(), C.c_double()
-            _check(lib().cf_results_condition(res, i, C.byref(st), C.byref(tr), C.byref(wall)))
-            metrics = {}
-            for name in names:
-                m = Metric()
-                if lib().cf_results_metric(res, i, name.encode(), confidence, C.byref(m)) == CF_OK:
-                    metrics[name] = m
-            out.append(ConditionResult(i, conditions[i] if i < len(conditions) else {}, STATUS[st.value],
-                                       tr.value, wall.value, metrics, confidence))
+        return [_result(res, i, keys, confidence) for i in range(lib().cf_results_count(res))]
     finally:
         lib().cf_results_free(res)
-    return out
 
 
 def _conditions(grid):
@@ -395,11 +480,133 @@ def _conditions(grid):
     return conds
 
 
+class ProgressBar:
+    """callbacks.rs:85-166: a line per condition on the standard error, and a progress line
+    within a condition every `every_s` seconds."""
+
+    def __init__(self, every_s=5.0):
+        self.every_s = float(every_s)
+
+    def _install(self, hooks):
+        _check(lib().cf_hooks_progress_bar(hooks, self.every_s))
+
+
+class SkipAfter:
+    """callbacks.rs:168-203: once `metric` falls below `threshold`, skip the later conditions
+    equal to that one on every key but `along`."""
+
+    def __init__(self, metric, threshold, along):
+        self.metric, self.threshold, self.along = metric, float(threshold), along
+
+    @staticmethod
+    def below(metric, threshold, along):
+        return SkipAfter(metric, threshold, along)
+
+    def _install(self, hooks):
+        _check(lib().cf_hooks_skip_after(hooks, self.metric.encode(), self.threshold, self.along.encode()))
+
+
+class CsvLogger:
+    """loggers.rs:107-215: one CSV row per condition, written and flushed as it ends."""
+
+    def __init__(self, path):
+        self.path = os.fspath(path)
+
+    def _install(self, hooks):
+        _check(lib().cf_hooks_csv_logger(hooks, self.path.encode()))
+
+
+class JsonLogger:
+    """loggers.rs:221-318: the full report as JSON at the end of the run."""
+
+    def __init__(self, path):
+        self.path = os.fspath(path)
+
+    def _install(self, hooks):
+        _check(lib().cf_hooks_json_logger(hooks, self.path.encode()))
+
+
+class _Hooks:
+    """The native hooks of one run: built-in observers, and adapters for Python objects with
+    any of on_run_start(info), on_condition_start(index, condition) -> Action,
+    on_batch_end(progress) -> Action, on_condition_end(result) -> Action, on_run_end(results)
+    (a callback) or start(info), log(result, confidence), finish(info, results, wall_s) (a
+    logger). An exception in Python ends the run (the condition is skipped or stopped, the
+    run stopped, a logger fails) and is raised once the run returns."""
+
+    def __init__(self, callbacks, loggers, keys, confidence):
+        self.errors, self.keep = [], []
+        self.h = C.c_void_p()
+        _check(lib().cf_hooks_create(C.byref(self.h)))
+        try:
+            for c in callbacks:
+                c._install(self.h) if hasattr(c, "_install") else self._callback(c, keys, confidence)
+            for l in loggers:
+                l._install(self.h) if hasattr(l, "_install") else self._logger(l, keys, confidence)
+        except BaseException:
+            self.close()
+            raise
+
+    def _guard(self, f, on_error):
+        def g(*a):
+            try:
+                return f(*a)
+            except BaseException as e:  # noqa: BLE001 - carried across the C boundary
+                self.errors.append(e)
+                return on_error
+        return g
+
+    def _callback(self, obj, keys, conf):
+        fn = lambda name: getattr(obj, name, None)
+        act = lambda r: Action.CONTINUE if r is None else int(r)
+        rs, cs, be, ce, re_ = (fn("on_run_start"), fn("on_condition_start"), fn("on_batch_end"),
+                               fn("on_condition_end"), fn("on_run_end"))
+        t = CallbackC()
+        if rs:
+            t.on_run_start = ON_RUN_START(self._guard(lambda u, i: rs(RunInfo(i.contents)), None))
+        if cs:
+            t.on_condition_start = ON_CONDITION_START(self._guard(
+                lambda u, i, v, n: act(cs(i, {keys[q]: v[q] for q in range(n)})), Action.SKIP))
+        if be:
+            t.on_batch_end = ON_BATCH_END(self._guard(lambda u, p: act(be(Progress(p.contents))),
+                                                      Action.STOP_CONDITION))
+        if ce:
+            t.on_condition_end = ON_CONDITION_END(self._guard(
+                lambda u, r: act(ce(_result(r, 0, keys, conf))), Action.STOP_RUN))
+        if re_:
+            t.on_run_end = ON_RUN_END(self._guard(
+                lambda u, r: re_([_result(r, i, keys, conf) for i in range(lib().cf_results_count(r))]), None))
+        self.keep.append(t)
+        _check(lib().cf_hooks_add_callback(self.h, C.byref(t)))
+
+    def _logger(self, obj, keys, conf):
+        fn = lambda name: getattr(obj, name, None)
+        st, lg, fi = fn("start"), fn("log"), fn("finish")
+        ok = lambda f: (lambda *a: (f(*a), 0)[1])
+        t = LoggerC()
+        if st:
+            t.start = LOG_START(self._guard(ok(lambda u, i: st(RunInfo(i.contents))), 1))
+        if lg:
+            t.log = LOG_LOG(self._guard(ok(lambda u, r, c: lg(_result(r, 0, keys, c), c)), 1))
+        if fi:
+            t.finish = LOG_FINISH(self._guard(ok(lambda u, i, r, w: fi(
+                RunInfo(i.contents), [_result(r, q, keys, conf) for q in range(lib().cf_results_count(r))], w)), 1))
+        self.keep.append(t)
+        _check(lib().cf_hooks_add_logger(self.h, C.byref(t)))
+
+    def close(self):
+        if self.h:
+            lib().cf_hooks_destroy(self.h)
+            self.h = C.c_void_p()
+
+
 class Runner:
-    """runner.rs:188-473: seed, batch, stop, confidence, fast_dev_run; run(scenario, grid)."""
+    """runner.rs:188-473: seed, batch, threads, stop, confidence, fast_dev_run, config,
+    callbacks and loggers; run(scenario, grid)."""
 
     def __init__(self):
         self._seed, self._batch, self._stop, self._conf, self._dev = 0, 32, Stop.max_trials(100), 0.95, False
+        self._threads, self._config_json, self._callbacks, self._loggers = 0, None, [], []
 
     def seed(self, s):
         self._seed = s
@@ -407,6 +614,10 @@ class Runner:
 
     def batch(self, n):
         self._batch = max(1, int(n))
+        return self
+
+    def threads(self, n):
+        self._threads = int(n)
         return self
 
     def stop(self, rule):
@@ -421,16 +632,48 @@ class Runner:
         self._dev = bool(on)
         return self
 
-    def _config(self):
-        return RunnerConfig(self._seed, self._batch, self._conf, int(self._dev))
+    def config(self, json_doc):
+        self._config_json = str(json_doc)
+        return self
+
+    def callback(self, c):
+        self._callbacks.append(c)
+        return self
+
+    def logger(self, l):
+        self._loggers.append(l)
+        return self
+
+    def _config(self, hooks=None):
+        return RunnerConfig(self._seed, self._batch, self._conf, int(self._dev), self._threads,
+                            self._config_json.encode() if self._config_json is not None else None,
+                            hooks.h if hooks is not None else None)
 
     def _stop_arr(self):
         nodes = self._stop.nodes()
         arr = (StopNode * len(nodes))(*nodes)
         return arr, len(nodes)
 
-    def run(self, name, batch_fn, grid, metrics=("ber", "fer", "iterations")):
-        """Run a scenario given as batch_fn(index, condition: dict, t0, t1, record)."""
+    def _run(self, grid, call, errors=()):
+        keys = [name for name, _ in grid.parts]
+        hooks = _Hooks(self._callbacks, self._loggers, keys, self._conf)
+        try:
+            res = C.c_void_p()
+            status = call(self._config(hooks), C.byref(res))
+            failures = list(errors) + hooks.errors
+            if failures:
+                if status == CF_OK:
+                    lib().cf_results_free(res)
+                raise failures[0]
+            _check(status)
+            return _collect(res, keys, self._conf)
+        finally:
+            hooks.close()
+
+    def run(self, name, batch_fn, grid, metrics=None):
+        """Run a scenario given as batch_fn(index, condition: dict, t0, t1, record); every
+        metric the scenario records is in the results (`metrics` is accepted for older
+        callers and ignored)."""
         errors = []
         conds = _conditions(grid)
 
@@ -449,17 +692,11 @@ class Runner:
         fn = BATCH_FN(cb)
         axes, keep = grid._axes()
         stop, n_stop = self._stop_arr()
-        cfg = self._config()
-        res = C.c_void_p()
-        status = lib().cf_run(C.byref(cfg), stop, n_stop, axes, len(grid.parts), name.encode(), fn, None,
-                              C.byref(res))
-        if errors:
-            raise errors[0]
-        _check(status)
-        return _collect(res, conds, self._conf, metrics)
+        return self._run(grid, lambda cfg, out: lib().cf_run(C.byref(cfg), stop, n_stop, axes, len(grid.parts),
+                                                             name.encode(), fn, None, out), errors)
 
     def run_nr(self, code, grid, modulation=BPSK, snr_kind=EBN0_DB, learned_params=None, fallback=False,
-               metrics=("ber", "fer", "iterations")):
+               metrics=None):
         """Runner::run with the NR link on the GPU as the scenario (cf_nr_campaign). With
         learned_params (iterations, rows, 2), the learned decoder runs its iterations and, with
         fallback, the code's own min-sum on the codewords whose syndrome still fails."""
@@ -467,11 +704,8 @@ class Runner:
                       else None, learned_params.shape[0] if learned_params is not None else 0, int(fallback))
         axes, keep = grid._axes()
         stop, n_stop = self._stop_arr()
-        cfg = self._config()
-        res = C.c_void_p()
-        _check(lib().cf_nr_campaign(code._h, C.byref(link), C.byref(cfg), stop, n_stop, axes, len(grid.parts),
-                                    C.byref(res)))
-        return _collect(res, _conditions(grid), self._conf, metrics)
+        return self._run(grid, lambda cfg, out: lib().cf_nr_campaign(code._h, C.byref(link), C.byref(cfg), stop,
+                                                                     n_stop, axes, len(grid.parts), out))
 
 
 def wilson(errors, total, confidence=0.95):

@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <iterator>
 #include <memory>
 #include <string>
 #include <vector>
@@ -14,17 +15,6 @@
 #include "kernels.hpp"
 #include "nr_code.hpp"
 #include "sim.hpp"
-
-struct cf_record {
-    comfec::sim::Record rec;
-    bool failed = false;
-    int kind = 0;
-    std::string message;
-};
-
-struct cf_results {
-    std::vector<comfec::sim::ConditionResult> results;
-};
 
 using namespace comfec;
 using namespace comfec::detail;
@@ -115,6 +105,15 @@ TrialGenArgs make_generator_args(const NrCode& c, int device, int modulation, in
 
 namespace {
 
+cf_metric metric_of(const sim::Record& rec, const char* name, double confidence) {
+    const sim::Estimate* e = rec.get(name);
+    if (!e) fail(Kind::Condition, std::string("no metric `") + name + "` in the result");
+    auto [lo, hi] = e->interval(confidence);
+    auto d = rec.dropped.find(name);
+    return cf_metric{e->value(), lo, hi, e->count(), e->total(), e->is_rate ? 1 : 0,
+                     d == rec.dropped.end() ? 0 : d->second};
+}
+
 sim::Stop stop_from_nodes(const cf_stop_node* nodes, size_t n, size_t& pos) {
     if (pos >= n) fail(Kind::Config, "the stopping rule ends early");
     const cf_stop_node& s = nodes[pos++];
@@ -139,6 +138,12 @@ sim::Runner make_runner(const cf_runner_config* config, const cf_stop_node* stop
     if (!config) fail(Kind::Config, "null runner configuration");
     sim::Runner r;
     r.seed(config->seed).batch(config->batch).confidence(config->confidence).fast_dev_run(config->fast_dev_run != 0);
+    r.threads(size_t(config->threads));
+    if (config->config_json) r.config(config->config_json);
+    if (config->hooks) {
+        for (const auto& c : config->hooks->callbacks) r.callback(c);
+        for (const auto& l : config->hooks->loggers) r.logger(l);
+    }
     if (stop && n_stop) {
         size_t pos = 0;
         r.stop(stop_from_nodes(stop, n_stop, pos));
@@ -303,6 +308,7 @@ cf_status cf_generate_batch(const cf_code* h, int modulation, int snr_kind, doub
 cf_status cf_record_errors(cf_record* rec, const char* name, uint64_t errors, uint64_t total) {
     return guarded([&] {
         if (!rec || !name) fail(Kind::Config, "null record or name");
+        if (rec->view) fail(Kind::Config, "the record of a progress report is read-only");
         rec->rec.errors(name, errors, total);
     });
 }
@@ -310,6 +316,7 @@ cf_status cf_record_errors(cf_record* rec, const char* name, uint64_t errors, ui
 cf_status cf_record_value(cf_record* rec, const char* name, double value) {
     return guarded([&] {
         if (!rec || !name) fail(Kind::Config, "null record or name");
+        if (rec->view) fail(Kind::Config, "the record of a progress report is read-only");
         rec->rec.value(name, value);
     });
 }
@@ -364,13 +371,43 @@ cf_status cf_results_metric(const cf_results* res, size_t i, const char* name, d
     return guarded([&] {
         if (!res || i >= res->results.size()) fail(Kind::Config, "no condition " + std::to_string(i));
         if (!name || !out) fail(Kind::Config, "null name or output");
-        const auto& rec = res->results[i].record;
-        const sim::Estimate* e = rec.get(name);
-        if (!e) fail(Kind::Condition, std::string("no metric `") + name + "` in the result");
-        auto [lo, hi] = e->interval(confidence);
-        auto d = rec.dropped.find(name);
-        *out = cf_metric{e->value(), lo, hi, e->count(), e->total(), e->is_rate ? 1 : 0,
-                         d == rec.dropped.end() ? 0 : d->second};
+        *out = metric_of(res->results[i].record, name, confidence);
+    });
+}
+
+cf_status cf_record_metric(const cf_record* rec, const char* name, double confidence, cf_metric* out) {
+    return guarded([&] {
+        if (!rec || !name || !out) fail(Kind::Config, "null record, name or output");
+        *out = metric_of(rec->get(), name, confidence);
+    });
+}
+
+cf_status cf_results_values(const cf_results* res, size_t i, size_t* index, double* values, size_t cap,
+                            size_t* n_values) {
+    return guarded([&] {
+        if (!res || i >= res->results.size()) fail(Kind::Config, "no condition " + std::to_string(i));
+        const auto& r = res->results[i];
+        if (index) *index = r.index;
+        const auto& e = r.condition.entries;
+        for (size_t j = 0; j < e.size() && j < cap && values; ++j) values[j] = e[j].second;
+        if (n_values) *n_values = e.size();
+    });
+}
+
+cf_status cf_results_metric_count(const cf_results* res, size_t i, size_t* n) {
+    return guarded([&] {
+        if (!res || i >= res->results.size()) fail(Kind::Config, "no condition " + std::to_string(i));
+        if (!n) fail(Kind::Config, "null output");
+        *n = res->results[i].record.metrics.size();
+    });
+}
+
+cf_status cf_results_metric_name(const cf_results* res, size_t i, size_t j, const char** name) {
+    return guarded([&] {
+        if (!res || i >= res->results.size()) fail(Kind::Config, "no condition " + std::to_string(i));
+        const auto& m = res->results[i].record.metrics;
+        if (!name || j >= m.size()) fail(Kind::Config, "no metric " + std::to_string(j));
+        *name = std::next(m.begin(), std::ptrdiff_t(j))->first.c_str();
     });
 }
 

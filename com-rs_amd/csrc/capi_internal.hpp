@@ -2,12 +2,15 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/comfec_gpu.h"
 #include "kernels.hpp"
 #include "nr_code.hpp"
+#include "sim.hpp"
 
 struct cf_code {
     comfec::NrCode code;
@@ -19,6 +22,24 @@ struct cf_code {
     }
 };
 
+// A scenario's record of a batch, or (view set) a read-only view of a condition's record.
+struct cf_record {
+    comfec::sim::Record rec;
+    const comfec::sim::Record* view = nullptr;
+    bool failed = false;
+    int kind = 0;
+    std::string message;
+    const comfec::sim::Record& get() const { return view ? *view : rec; }
+};
+
+struct cf_results {
+    std::vector<comfec::sim::ConditionResult> results;
+};
+
+struct cf_hooks {
+    std::vector<std::shared_ptr<comfec::sim::Callback>> callbacks;
+    std::vector<std::shared_ptr<comfec::sim::Logger>> loggers;
+};
 
 namespace comfec {
 namespace detail {

@@ -5,27 +5,22 @@
 #include <algorithm>
 #include <charconv>
 #include <cmath>
+#include <ctime>
 #include <limits>
+#include <thread>
 
 #include "nr_code.hpp"
 
 namespace comfec {
 namespace sim {
 
-namespace {
-
-// Rust's Display of an f64: the shortest decimal that reads back to the value, without
-// an exponent and without a trailing ".0".
-std::string fmt(double v) {
+std::string display(double v) {
     if (std::isnan(v)) return "NaN";
     if (std::isinf(v)) return v > 0 ? "inf" : "-inf";
-    char buf[64];
+    char buf[400];
     auto r = std::to_chars(buf, buf + sizeof buf, v, std::chars_format::fixed);
-    std::string s(buf, r.ptr);
-    return s;
+    return std::string(buf, r.ptr);
 }
-
-}  // namespace
 
 // crates/com-math/src/special.rs:180-255: Cody's rational Chebyshev approximation (CALERF)
 double erfc_cody(double x) {
@@ -200,7 +195,7 @@ std::string Condition::label() const {
     std::string s;
     for (size_t i = 0; i < entries.size(); ++i) {
         if (i) s += ", ";
-        s += entries[i].first + "=" + fmt(entries[i].second);
+        s += entries[i].first + "=" + display(entries[i].second);
     }
     return s;
 }
@@ -396,7 +391,7 @@ std::string Stop::str() const {
         case MaxTrials: return "max_trials(" + std::to_string(count) + ")";
         case MinTrials: return "min_trials(" + std::to_string(count) + ")";
         case MinErrors: return "errors(" + metric + ") ≥ " + std::to_string(count);
-        case Precision: return "precision(" + metric + ") ≤ " + fmt(value) + " at " + fmt(confidence);
+        case Precision: return "precision(" + metric + ") ≤ " + display(value) + " at " + display(confidence);
         case MaxTime: {
             char buf[64];
             std::snprintf(buf, sizeof buf, "time ≥ %.1f s", value);
@@ -427,9 +422,95 @@ Error with_context(const Error& e, const std::string& context) {
 void check_confidence(double c, const std::string& what) {
     if (!(c > 0.0 && c < 1.0))
         fail(Kind::Config,
-             what + ": the confidence level must be between 0 and 1 (0.95 for 95 %), got " + fmt(c));
+             what + ": the confidence level must be between 0 and 1 (0.95 for 95 %), got " + display(c));
 }
 
+const char* status_str(Status s) {
+    switch (s) {
+        case Status::Completed: return "completed";
+        case Status::Stopped: return "stopped";
+        case Status::Skipped: return "skipped";
+        case Status::DevRun: return "dev_run";
+    }
+    return "";
+}
+
+RunInfo Runner::run_info(const std::string& scenario, size_t n_conditions) const {
+    RunInfo i;
+    i.scenario = scenario;
+    i.seed = seed_;
+    i.batch = batch_;
+    i.threads = threads_ ? threads_ : std::max(1u, std::thread::hardware_concurrency());
+    i.stop = fast_dev_run_ ? "fast_dev_run" : stop_.str();
+    i.confidence = confidence_;
+    i.n_conditions = n_conditions;
+    i.started_unix_s = uint64_t(std::max<std::time_t>(std::time(nullptr), 0));
+    i.version = kVersion;
+    i.commit = kCommit;
+    i.has_config = has_config_;
+    i.config = config_;
+    return i;
+}
+
+// runner.rs:417-473: batches of trials until the rule is met (or a callback stops the
+// condition), the last batch ending at the rule's bound on the trials
+ConditionResult Runner::run_condition(BatchScenario& scenario, size_t index, const Condition& cond) {
+    const auto t0 = std::chrono::steady_clock::now();
+    try {
+        scenario.setup(cond, index);
+    } catch (const Error& e) {
+        throw with_context(e, "setup of (" + cond.label() + ")");
+    }
+    ConditionResult res;
+    res.index = index;
+    res.condition = cond;
+    uint64_t limit = 0;
+    const bool bounded = !fast_dev_run_ && stop_.trial_limit(limit);
+    uint64_t trials = 0;
+    for (;;) {
+        const uint64_t end = bounded ? std::max(std::min(trials + batch_, limit), trials + 1) : trials + batch_;
+        try {
+            scenario.run_batch(cond, index, trials, end, res.record);
+        } catch (const Error& e) {
+            throw with_context(e, "trials " + std::to_string(trials) + ".." + std::to_string(end) + " of (" +
+                                      cond.label() + ")");
+        }
+        if (trials == 0) {
+            try {
+                stop_.check_metrics(res.record);
+            } catch (const Error& e) {
+                throw with_context(e, "(" + cond.label() + ")");
+            }
+        }
+        trials = end;
+        const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        Progress p{index, trials, &res.record, elapsed};
+        // every callback sees the batch; Skip after a batch stops the condition too
+        bool stopped = false;
+        for (auto& c : callbacks_) {
+            const Action a = c->on_batch_end(p);
+            if (a == Action::StopCondition || a == Action::Skip) stopped = true;
+        }
+        if (fast_dev_run_) {
+            res.status = Status::DevRun;
+            break;
+        }
+        if (stop_.is_met(p)) {
+            res.status = Status::Completed;
+            break;
+        }
+        if (stopped) {
+            res.status = Status::Stopped;
+            break;
+        }
+    }
+    res.trials = trials;
+    res.wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return res;
+}
+
+// runner.rs:309-415: the callbacks, then the loggers, at the start of the run, at the end
+// of every condition (run or skipped) and at the end of the run
 std::vector<ConditionResult> Runner::run(BatchScenario& scenario, const Grid& grid) {
     check_confidence(confidence_, "the runner");
     stop_.check_confidences();
@@ -437,54 +518,39 @@ std::vector<ConditionResult> Runner::run(BatchScenario& scenario, const Grid& gr
     if (!fast_dev_run_ && !stop_.is_bounded())
         fail(Kind::Config, "the stopping rule `" + stop_.str() +
                                "` can run forever: add Stop::max_trials or Stop::max_time with `or`");
-    std::vector<ConditionResult> results;
     const auto conds = grid.conditions();
+    const RunInfo info = run_info(scenario.name(), conds.size());
+    for (auto& c : callbacks_) c->on_run_start(info);
+    for (auto& l : loggers_) l->start(info);
+    const auto t_run = std::chrono::steady_clock::now();
+    std::vector<ConditionResult> results;
+    results.reserve(conds.size());
+    bool stop_run = false;
     for (size_t index = 0; index < conds.size(); ++index) {
         const Condition& cond = conds[index];
-        const auto t0 = std::chrono::steady_clock::now();
-        try {
-            scenario.setup(cond, index);
-        } catch (const Error& e) {
-            throw with_context(e, "setup of (" + cond.label() + ")");
-        }
+        // every callback sees the start of the condition unless the run was stopped
+        bool skip = stop_run;
+        if (!stop_run)
+            for (auto& c : callbacks_)
+                if (c->on_condition_start(index, cond) == Action::Skip) skip = true;
         ConditionResult res;
-        res.index = index;
-        res.condition = cond;
-        uint64_t limit = 0;
-        const bool bounded = !fast_dev_run_ && stop_.trial_limit(limit);
-        uint64_t trials = 0;
-        for (;;) {
-            const uint64_t end = bounded ? std::max(std::min(trials + batch_, limit), trials + 1) : trials + batch_;
-            try {
-                scenario.run_batch(cond, index, trials, end, res.record);
-            } catch (const Error& e) {
-                throw with_context(e, "trials " + std::to_string(trials) + ".." + std::to_string(end) + " of (" +
-                                          cond.label() + ")");
-            }
-            if (trials == 0) {
-                try {
-                    stop_.check_metrics(res.record);
-                } catch (const Error& e) {
-                    throw with_context(e, "(" + cond.label() + ")");
-                }
-            }
-            trials = end;
-            const double elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            Progress p{index, trials, &res.record, elapsed};
-            if (fast_dev_run_) {
-                res.status = Status::DevRun;
-                break;
-            }
-            if (stop_.is_met(p)) {
-                res.status = Status::Completed;
-                break;
-            }
+        if (skip) {
+            res.index = index;
+            res.condition = cond;
+            res.status = Status::Skipped;
+        } else {
+            res = run_condition(scenario, index, cond);
         }
-        res.trials = trials;
-        res.wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (auto& c : callbacks_)
+            if (c->on_condition_end(res) == Action::StopRun) stop_run = true;
+        for (auto& l : loggers_) l->log(res, confidence_);
         results.push_back(std::move(res));
     }
-    return results;
+    for (auto& c : callbacks_) c->on_run_end(results);
+    RunReport report{info, std::move(results),
+                     std::chrono::duration<double>(std::chrono::steady_clock::now() - t_run).count()};
+    for (auto& l : loggers_) l->finish(report);
+    return std::move(report.results);
 }
 
 }  // namespace sim

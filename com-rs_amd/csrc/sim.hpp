@@ -6,6 +6,7 @@
 #pragma once
 #include <chrono>
 #include <cstdint>
+#include <cstdio>
 #include <functional>
 #include <map>
 #include <memory>
@@ -17,6 +18,10 @@
 
 namespace comfec {
 namespace sim {
+
+// Rust's Display of an f64: the shortest decimal that reads back to the value, with no
+// exponent and no trailing ".0" (grid labels, stopping rules, JSON numbers).
+std::string display(double v);
 
 // z = Q^-1(alpha/2) of a two-sided level (stats.rs:43-45), through Cody's erfc
 // (special.rs:180-255) and its bisection inverse (special.rs:265-273, search.rs:69-80).
@@ -143,6 +148,7 @@ class BatchScenario {
 };
 
 enum class Status { Completed = 0, Stopped = 1, Skipped = 2, DevRun = 3 };
+const char* status_str(Status s);  // Status::as_str (runner.rs:70-77)
 
 // runner.rs:71-99
 struct ConditionResult {
@@ -153,6 +159,138 @@ struct ConditionResult {
     double wall_s = 0.0;
     Record record;
 };
+
+// ConditionResult::row (runner.rs:106-137): index, the condition, status, trials, wall_s,
+// then per metric (in name order) the estimate, its interval and its two counts
+struct Cell {
+    enum Type { F64, I64, Str } type = F64;
+    double f = 0.0;
+    int64_t i = 0;
+    std::string s;
+};
+std::vector<std::pair<std::string, Cell>> row(const ConditionResult& r, double confidence);
+
+// RunInfo (runner.rs:150-175)
+struct RunInfo {
+    std::string scenario;
+    uint64_t seed = 0, batch = 0;
+    size_t threads = 0;
+    std::string stop;
+    double confidence = 0.95;
+    size_t n_conditions = 0;
+    uint64_t started_unix_s = 0;
+    std::string version, commit;
+    bool has_config = false;
+    std::string config;  // a JSON document
+};
+
+// RunReport (runner.rs:177-186)
+struct RunReport {
+    RunInfo info;
+    std::vector<ConditionResult> results;
+    double wall_s = 0.0;
+};
+
+// Action (callbacks.rs:50-61)
+enum class Action { Continue = 0, Skip = 1, StopCondition = 2, StopRun = 3 };
+
+// Callback (callbacks.rs:64-81): every hook optional
+class Callback {
+  public:
+    virtual ~Callback() = default;
+    virtual void on_run_start(const RunInfo&) {}
+    virtual Action on_condition_start(size_t, const Condition&) { return Action::Continue; }
+    virtual Action on_batch_end(const Progress&) { return Action::Continue; }
+    virtual Action on_condition_end(const ConditionResult&) { return Action::Continue; }
+    virtual void on_run_end(const std::vector<ConditionResult>&) {}
+};
+
+// Logger (loggers.rs:53-82): failures are Kind::Io errors that end the run
+class Logger {
+  public:
+    virtual ~Logger() = default;
+    virtual void start(const RunInfo&) {}
+    virtual void log(const ConditionResult& r, double confidence) = 0;
+    virtual void finish(const RunReport&) {}
+};
+
+// ProgressBar (callbacks.rs:85-166): a line per condition on the standard error, and a
+// progress line within a condition every `every_s` seconds
+class ProgressBar : public Callback {
+  public:
+    explicit ProgressBar(double every_s = 5.0, double confidence = 0.95) : every_s_(every_s), confidence_(confidence) {}
+    void on_run_start(const RunInfo& info) override;
+    Action on_condition_start(size_t, const Condition&) override;
+    Action on_batch_end(const Progress& p) override;
+    Action on_condition_end(const ConditionResult& r) override;
+
+  private:
+    double every_s_, confidence_;
+    size_t n_ = 0;
+    bool started_ = false;
+    std::chrono::steady_clock::time_point last_;
+};
+
+// SkipAfter::below (callbacks.rs:168-203): once `metric` falls below `threshold`, the later
+// conditions equal to that one on every key but `along` are skipped
+class SkipAfter : public Callback {
+  public:
+    SkipAfter(std::string metric, double threshold, std::string along)
+        : metric_(std::move(metric)), threshold_(threshold), along_(std::move(along)) {}
+    void on_run_start(const RunInfo&) override { done_.clear(); }
+    Action on_condition_start(size_t, const Condition& c) override;
+    Action on_condition_end(const ConditionResult& r) override;
+
+  private:
+    std::vector<std::pair<std::string, double>> others(const Condition& c) const;
+    std::string metric_;
+    double threshold_;
+    std::string along_;
+    std::vector<std::vector<std::pair<std::string, double>>> done_;
+};
+
+// CsvLogger (loggers.rs:107-215): one row per condition as it ends, flushed
+class CsvLogger : public Logger {
+  public:
+    explicit CsvLogger(std::string path) : path_(std::move(path)) {}
+    ~CsvLogger() override;
+    void start(const RunInfo&) override;
+    void log(const ConditionResult& r, double confidence) override;
+    void finish(const RunReport&) override;
+
+  private:
+    void write_header(const ConditionResult& r, double confidence);
+    void write_row(const ConditionResult& r, double confidence);
+    void put(const std::string& line);
+    std::string path_;
+    std::FILE* out_ = nullptr;
+    bool header_ = false;
+    std::vector<std::string> keys_, columns_;
+    std::vector<std::pair<ConditionResult, double>> pending_;
+};
+
+// JsonLogger (loggers.rs:221-318): the whole report at the end of the run
+class JsonLogger : public Logger {
+  public:
+    explicit JsonLogger(std::string path) : path_(std::move(path)) {}
+    void log(const ConditionResult&, double) override {}
+    void finish(const RunReport& rep) override;
+
+  private:
+    std::string path_;
+};
+
+// json.rs and loggers.rs helpers
+std::string csv_field(const std::string& s);
+std::string json_string(const std::string& s);
+std::string json_number(double v);
+// Rust's LowerExp of an f64 (`{:e}`: shortest digits; `{:.Ne}`: N digits after the point)
+std::string lower_exp(double v, int precision = -1);
+
+// The library's version and the commit it was built from (`<hash>`, `<hash>-dirty`, or
+// `unknown`), recorded in every run (crates/com-sim/src/lib.rs:81-88)
+extern const char* const kVersion;
+extern const char* const kCommit;
 
 // runner.rs:188-473
 class Runner {
@@ -177,14 +315,40 @@ class Runner {
         fast_dev_run_ = on;
         return *this;
     }
+    // worker threads reported in the run information (0: the host's cores); the trials of
+    // a batch run wherever the scenario runs them
+    Runner& threads(size_t n) {
+        threads_ = n;
+        return *this;
+    }
+    Runner& config(std::string json) {
+        config_ = std::move(json);
+        has_config_ = true;
+        return *this;
+    }
+    Runner& callback(std::shared_ptr<Callback> c) {
+        callbacks_.push_back(std::move(c));
+        return *this;
+    }
+    Runner& logger(std::shared_ptr<Logger> l) {
+        loggers_.push_back(std::move(l));
+        return *this;
+    }
     uint64_t seed_value() const { return seed_; }
+    RunInfo run_info(const std::string& scenario, size_t n_conditions) const;  // runner.rs:278-295
     std::vector<ConditionResult> run(BatchScenario& scenario, const Grid& grid);
 
   private:
+    ConditionResult run_condition(BatchScenario& scenario, size_t index, const Condition& cond);
     uint64_t seed_ = 0, batch_ = 32;
     Stop stop_ = Stop::max_trials(100);
     double confidence_ = 0.95;
     bool fast_dev_run_ = false;
+    size_t threads_ = 0;
+    bool has_config_ = false;
+    std::string config_;
+    std::vector<std::shared_ptr<Callback>> callbacks_;
+    std::vector<std::shared_ptr<Logger>> loggers_;
 };
 
 void check_confidence(double c, const std::string& what);

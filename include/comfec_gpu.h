@@ -227,12 +227,19 @@ typedef struct {
     size_t n_values;
 } cf_grid_axis;
 
+/* The callbacks and loggers of a run (Runner::callback, Runner::logger at runner.rs:266-276),
+   in the order added; see "callbacks and loggers" below. */
+typedef struct cf_hooks cf_hooks;
+
 /* The Runner builder (crates/com-sim/src/runner.rs:224-276). */
 typedef struct {
     uint64_t seed;
     uint64_t batch;
     double confidence;
     int fast_dev_run;
+    uint64_t threads;         /* Runner::threads, reported in the run information (0: the host's cores) */
+    const char* config_json;  /* Runner::config: a JSON document for the report, or NULL */
+    cf_hooks* hooks;          /* NULL: none */
 } cf_runner_config;
 
 /* Scenario::setup + Scenario::trial for the trials [t0, t1) of condition `index` whose
@@ -284,7 +291,76 @@ cf_status cf_wilson(uint64_t errors, uint64_t total, double confidence, double* 
 
 cf_status cf_results_metric(const cf_results* res, size_t i, const char* name, double confidence,
                             cf_metric* out);
+/* The grid index of result i and its condition's values in axis order (up to `cap` of them;
+   *n_values = their number). */
+cf_status cf_results_values(const cf_results* res, size_t i, size_t* index, double* values, size_t cap,
+                            size_t* n_values);
+/* The metrics of result i in name order (Record::iter, scenario.rs:189-192). */
+cf_status cf_results_metric_count(const cf_results* res, size_t i, size_t* n);
+cf_status cf_results_metric_name(const cf_results* res, size_t i, size_t j, const char** name);
 void cf_results_free(cf_results* res);
+
+/* ---- callbacks and loggers: crates/com-sim/src/callbacks.rs, loggers.rs ------------------- */
+
+/* Action (callbacks.rs:50-61) */
+enum { CF_CONTINUE = 0, CF_SKIP = 1, CF_STOP_CONDITION = 2, CF_STOP_RUN = 3 };
+
+/* RunInfo (runner.rs:150-175); the strings live for the duration of the call. */
+typedef struct {
+    const char* scenario;
+    uint64_t seed, batch, threads;
+    const char* stop;      /* the rule's Display, or "fast_dev_run" */
+    double confidence;
+    uint64_t n_conditions;
+    uint64_t started_unix_s;
+    const char* version;
+    const char* commit;    /* "<hash>", "<hash>-dirty" or "unknown" */
+    const char* config;    /* the Runner::config JSON, or NULL */
+} cf_run_info;
+
+/* Progress (stopping.rs:48-59): the condition's merged record after a batch. */
+typedef struct {
+    uint64_t condition;
+    uint64_t trials;
+    double elapsed_s;
+    const cf_record* record;
+} cf_progress;
+/* Record::get (scenario.rs:185-187) with the estimate's interval at `confidence`. */
+cf_status cf_record_metric(const cf_record* rec, const char* name, double confidence, cf_metric* out);
+
+/* Callback (callbacks.rs:64-81). Every function may be NULL (Continue). The results given to
+   on_condition_end hold the one condition that ended (read it with cf_results_*, index 0);
+   those of on_run_end hold every condition. Neither may be freed by the callee. */
+typedef struct {
+    void* user;
+    void (*on_run_start)(void* user, const cf_run_info* info);
+    int (*on_condition_start)(void* user, size_t index, const double* values, size_t n_values);
+    int (*on_batch_end)(void* user, const cf_progress* progress);
+    int (*on_condition_end)(void* user, const cf_results* result);
+    void (*on_run_end)(void* user, const cf_results* results);
+} cf_callback;
+
+/* Logger (loggers.rs:53-82). A nonzero return ends the run with CF_IO. */
+typedef struct {
+    void* user;
+    int (*start)(void* user, const cf_run_info* info);
+    int (*log)(void* user, const cf_results* result, double confidence);
+    int (*finish)(void* user, const cf_run_info* info, const cf_results* results, double wall_s);
+} cf_logger;
+
+cf_status cf_hooks_create(cf_hooks** out);
+/* Runner::callback / Runner::logger with a caller's hooks (copied; `user` is not owned). */
+cf_status cf_hooks_add_callback(cf_hooks* h, const cf_callback* callback);
+cf_status cf_hooks_add_logger(cf_hooks* h, const cf_logger* logger);
+/* ProgressBar::new (callbacks.rs:101-105): lines on the standard error. */
+cf_status cf_hooks_progress_bar(cf_hooks* h, double every_s);
+/* SkipAfter::below (callbacks.rs:178-183). */
+cf_status cf_hooks_skip_after(cf_hooks* h, const char* metric, double threshold, const char* along);
+/* CsvLogger::new (loggers.rs:117-128) and JsonLogger::new (loggers.rs:225-229). */
+cf_status cf_hooks_csv_logger(cf_hooks* h, const char* path);
+cf_status cf_hooks_json_logger(cf_hooks* h, const char* path);
+/* A hooks object serves one run at a time (its callbacks keep state within a run). */
+void cf_hooks_destroy(cf_hooks* h);
 
 /* ---- the learned offset/scale min-sum (no reference counterpart) ---------------------------
    The layered min-sum of ldpc.rs:509-531 whose check-node magnitude map, at iteration t and

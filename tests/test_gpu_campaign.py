@@ -142,3 +142,31 @@ def test_campaign_with_the_learned_decoder_and_its_fallback():
         extra = (c.metric("iterations").value - a.metric("iterations").value) * 4096 / 12
         assert abs(extra - round(extra)) < 1e-6 * 4096  # whole frames of 12 more iterations
         assert round(extra) >= a.metric("fer").count > 0
+
+
+def test_campaign_reports_and_skip_after(tmp_path):
+    """The GPU campaign with the runner's observers: SkipAfter ends the SNR sweep once the
+    FER is below 1e-3 (callbacks.rs:168-203), and the CSV rows and the JSON report
+    (loggers.rs:107-318) hold the results the campaign returns."""
+    import json
+    print("\n[campaign] loggers and skip-after", flush=True)
+    code = F.NrLdpc.lifted(2, 52).set_decoder("normalized_min_sum", 0.75, True, 20)
+    grid = F.Grid().axis("ebn0_db", [1.0, 4.0, 5.0, 6.0])
+    csv_path, json_path = tmp_path / "c.csv", tmp_path / "c.json"
+    r = (F.Runner().seed(3).batch(2048).stop(F.Stop.max_trials(4096))
+         .callback(F.SkipAfter.below("fer", 1e-3, "ebn0_db"))
+         .logger(F.CsvLogger(csv_path)).logger(F.JsonLogger(json_path)).run_nr(code, grid))
+    assert [c.status for c in r] == ["completed", "completed", "skipped", "skipped"]
+    assert r[0].metric("fer").count > 0 and r[1].metric("fer").count == 0
+    lines = csv_path.read_text().splitlines()
+    assert lines[0] == ("index,ebn0_db,status,trials,wall_s," + ",".join(
+        f"{m}{s}" for m in ("ber", "fer", "iterations") for s in ("", "_lo", "_hi", "_count", "_total")))
+    assert len(lines) == 5 and lines[3].startswith("2,5,skipped,0,")
+    for c, line in zip(r[:2], lines[1:3]):
+        cells = line.split(",")
+        assert float(cells[5]) == c.metric("ber").value and int(cells[8]) == c.metric("ber").count
+        assert float(cells[10]) == c.metric("fer").value and int(cells[13]) == c.metric("fer").count
+    doc = json.loads(json_path.read_text())
+    assert doc["scenario"] == "nr_ldpc" and doc["seed"] == 3 and doc["batch"] == 2048
+    assert [x["status"] for x in doc["conditions"]] == [c.status for c in r]
+    assert doc["conditions"][0]["metrics"]["fer"]["errors"] == r[0].metric("fer").count
