@@ -23,7 +23,8 @@ bool layered_supported(int bg, int z) { return (bg == 1 || bg == 2) && z >= 1 &&
 size_t layered_lds_bytes(int bg, int z, int g, bool half) {
     const int cols = bg == 1 ? 68 : 52;  // base columns
     const size_t post = size_t(g) * size_t(cols) * size_t(z) * (half ? 2 : 4);
-    const size_t hd = (z % 32 == 0) ? size_t(g) * size_t(cols) * 2 * size_t(z / 32) * 4 : 0;
+    // the packed decisions, with a zero column after the last
+    const size_t hd = (z % 32 == 0) ? size_t(g) * size_t(cols + 1) * 2 * size_t(z / 32) * 4 : 0;
     return ((post + 3) & ~size_t(3)) + hd;
 }
 

@@ -66,32 +66,35 @@ constexpr int row_begin(int i) {
     return s;
 }
 
-// Whether layer I must wait for the layers before it: it shares a column with a layer
-// processed since the last barrier (reads after writes; and writes after reads, which are
+// Whether layer l must wait for the layers before it: it shares a column with a layer
+// processed since the last barrier (reads after writes, and writes after reads, which are
 // the same shared columns). Layers on disjoint columns commute, so they run without one.
 template <int BG>
-constexpr bool shares_column(int i, int l) {
-    for (int e = 0; e < Graph<BG>::edges; ++e)
-        if (Graph<BG>::t[e].i == i)
-            for (int f = 0; f < Graph<BG>::edges; ++f)
-                if (Graph<BG>::t[f].i == l && Graph<BG>::t[f].j == Graph<BG>::t[e].j) return true;
-    return false;
+struct Schedule {
+    bool barrier[Graph<BG>::rows];
+};
+
+template <int BG>
+constexpr Schedule<BG> make_schedule() {
+    constexpr int R = Graph<BG>::rows, C = Graph<BG>::cols, E = Graph<BG>::edges;
+    Schedule<BG> s{};
+    bool has[R][C] = {};
+    for (int e = 0; e < E; ++e) has[Graph<BG>::t[e].i][Graph<BG>::t[e].j] = true;
+    int first = 0;  // first layer after the last barrier
+    for (int l = 0; l < R; ++l) {
+        bool conflict = false;
+        for (int p = first; p < l; ++p)
+            for (int c = 0; c < C; ++c) conflict = conflict || (has[l][c] && has[p][c]);
+        s.barrier[l] = conflict;
+        if (conflict) first = l;
+    }
+    return s;
 }
 
 template <int BG>
-constexpr bool barrier_before(int layer) {
-    if (layer == 0) return false;
-    int first = 0;  // first layer after the last barrier
-    for (int l = 1; l <= layer; ++l) {
-        bool conflict = false;
-        for (int p = first; p < l; ++p) conflict = conflict || shares_column<BG>(l, p);
-        if (conflict) {
-            if (l == layer) return true;
-            first = l;
-        }
-    }
-    return false;
-}
+struct Sched {
+    static constexpr Schedule<BG> s = make_schedule<BG>();
+};
 
 // The lifting set of Z and the shift V mod Z of base entry e, when Z is known at compile
 // time (nr/ldpc.rs:117-133).
@@ -127,6 +130,9 @@ struct Store<false> {
     // at LDS byte address `off` (the kernel's posteriors start at LDS address 0)
     static __device__ __forceinline__ float ld_at(uint32_t off) { return *(const lds_ptr<float>::type)(uintptr_t)off; }
     static __device__ __forceinline__ void st_at(uint32_t off, float x) { *(lds_ptr<float>::type)(uintptr_t)off = x; }
+    static __device__ __forceinline__ uint32_t sign_at(uint32_t off) {
+        return *(const lds_ptr<uint32_t>::type)(uintptr_t)off >> 31;
+    }
 };
 
 __device__ __forceinline__ float h2f(uint32_t h) {
@@ -153,6 +159,9 @@ struct Store<true> {
     }
     static __device__ __forceinline__ void st_at(uint32_t off, float x) {
         *(lds_ptr<uint16_t>::type)(uintptr_t)off = to_store(x);
+    }
+    static __device__ __forceinline__ uint32_t sign_at(uint32_t off) {
+        return uint32_t(*(const lds_ptr<uint16_t>::type)(uintptr_t)off) >> 15;
     }
 };
 
@@ -222,7 +231,11 @@ struct Messages<ROWS, true> {
 //  - the edge that takes the second minimum is told by |input_q| == m1: it is the first
 //    minimum, or a tie with it, and a tie makes m2 = m1, so either way the magnitude is the
 //    reference's g(q == i1 ? m2 : m1);
-//  - m2 = med3(m1, m2, |input|) is the strict-< update of the scan of ldpc.rs:471-480.
+//  - m1 and m2 are the smallest and the second smallest input magnitude counted with
+//    multiplicity, which is what the strict-< scan of ldpc.rs:471-480 finds (a tie gives
+//    m2 = m1); they are selected, never computed, so the order of the comparisons does not
+//    matter: three inputs at a time give their min3 and med3, and a pair (t1 <= t2) merges
+//    into (m1 <= m2) as min(m1, t1) and med3(m1, t1, min(m2, t2)).
 // The stored state keeps the output sign bits (edge q at bit deg-1-q, shifted in with a
 // funnel shift) and one index whose magnitude is g(m2).
 template <int BG, bool HALF, bool SOLO, int ZC, int I>
@@ -253,39 +266,62 @@ __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post
     const uint32_t pko = msg.pk[I];
     const uint32_t i1o = pko >> 24;
     constexpr uint32_t es = sizeof(typename S::T);
-    // (r + shift) mod z for edge q: the unsigned min of t and t - z (which wraps when t < z);
-    // with ZC, the LDS byte address (the posteriors start at 0) so that the column base is
-    // the LDS instruction's immediate offset, and a zero shift (every extension column)
-    // needs no wrap; otherwise the element index
-    auto address = [&](int q) -> int {
+    // (r + shift) mod z of entry e: the unsigned min of t and t - z (which wraps when
+    // t < z); with ZC, the LDS byte address (the posteriors start at 0) so that the column
+    // base is the LDS instruction's immediate offset, and a zero shift (every extension
+    // column) needs no wrap; otherwise the element index
+    auto address_of = [&](int e) -> int {
         if constexpr (ZC != 0) {
-            const uint32_t s_q = shift_of<BG, ZC>(e0 + q) * es;
+            const uint32_t s_q = shift_of<BG, ZC>(e) * es;
             const uint32_t t = uint32_t(r) * es + s_q;
             const uint32_t rr = s_q == 0 ? t : min(t, t - uint32_t(ZC) * es);
-            return int(rr) + int(Graph<BG>::t[e0 + q].j) * ZC * int(es);
+            return int(rr) + int(Graph<BG>::t[e].j) * ZC * int(es);
         } else {
-            const uint32_t t = uint32_t(r) + shift[e0 + q];
+            const uint32_t t = uint32_t(r) + shift[e];
             const uint32_t rr = min(t, t - uint32_t(z));
-            return int(Graph<BG>::t[e0 + q].j) * z + int(rr) + int(base);
+            return int(Graph<BG>::t[e].j) * z + int(rr) + int(base);
         }
     };
+    auto address = [&](int q) -> int { return address_of(e0 + q); };
+    auto load = [&](int ad) -> float { return ZC != 0 ? S::ld_at(uint32_t(ad)) : S::ld(post, ad); };
     // FP16 kernels run three waves a SIMD (168 registers): their widest layers compute the
     // addresses twice rather than hold them
     constexpr bool keep = !HALF || deg <= 10;
     float v2c[deg];
     int addr[keep ? deg : 1];
-    float m1 = INFINITY, m2 = INFINITY;
-    uint32_t parity = 0;
 #pragma unroll
     for (int q = 0; q < deg; ++q) {
         const int ad = address(q);
         if constexpr (keep) addr[q] = ad;
         const float mag = uint32_t(q) == i1o ? g2o : g1o;
         const float old = __uint_as_float(__float_as_uint(mag) ^ ((pko << (32 - deg + q)) & 0x80000000u));
-        const float in = (ZC != 0 ? S::ld_at(uint32_t(ad)) : S::ld(post, ad)) - old;
-        v2c[q] = in;
-        parity ^= __float_as_uint(in);
-        const float m = fabsf(in);
+        v2c[q] = load(ad) - old;
+    }
+    // the row's sign product: three-input XORs
+    uint32_t parity = 0;
+#pragma unroll
+    for (int q = 0; q < deg; q += 2)
+        parity = q + 1 < deg ? __builtin_amdgcn_bitop3_b32(parity, __float_as_uint(v2c[q]),
+                                                           __float_as_uint(v2c[q + 1]), 0x96)
+                             : parity ^ __float_as_uint(v2c[q]);
+    // the two smallest magnitudes, three inputs at a time
+    float m1 = INFINITY, m2 = INFINITY;
+    constexpr int triples = deg / 3;
+#pragma unroll
+    for (int t = 0; t < triples; ++t) {
+        const float x = fabsf(v2c[3 * t]), y = fabsf(v2c[3 * t + 1]), w = fabsf(v2c[3 * t + 2]);
+        const float t1 = fminf(fminf(x, y), w), t2 = __builtin_amdgcn_fmed3f(x, y, w);
+        if (t == 0) {
+            m1 = t1;
+            m2 = t2;
+        } else {
+            m2 = __builtin_amdgcn_fmed3f(m1, t1, fminf(m2, t2));
+            m1 = fminf(m1, t1);
+        }
+    }
+#pragma unroll
+    for (int q = 3 * triples; q < deg; ++q) {
+        const float m = fabsf(v2c[q]);
         m2 = __builtin_amdgcn_fmed3f(m1, m2, m);
         m1 = fminf(m1, m);
     }
@@ -318,7 +354,7 @@ __device__ __forceinline__ void sweep(typename Store<HALF>::T* post, int r, uint
             a = ab[2 * I];
             b = ab[2 * I + 1];
         }
-        if constexpr (barrier_before<BG>(I)) __syncthreads();
+        if constexpr (Sched<BG>::s.barrier[I]) __syncthreads();
         // (the branch also bounds the scheduler's reach: without it the compiler interleaves
         // layers and spills hundreds of registers)
         if (active) layer<BG, HALF, SOLO, ZC, I>(post, r, gofs, z, shift, msg, a, b);
@@ -326,59 +362,86 @@ __device__ __forceinline__ void sweep(typename Store<HALF>::T* post, int r, uint
     }
 }
 
+template <int BG>
+constexpr int max_degree() {
+    int m = 0;
+    for (int i = 0; i < Graph<BG>::rows; ++i) m = row_begin<BG>(i + 1) - row_begin<BG>(i) > m ? row_begin<BG>(i + 1) - row_begin<BG>(i) : m;
+    return m;
+}
+constexpr int kSyndSlots = Graph<1>::rows * max_degree<1>() > Graph<2>::rows * max_degree<2>()
+                               ? Graph<1>::rows * max_degree<1>()
+                               : Graph<2>::rows * max_degree<2>();
+
 // The tables of the task's class in LDS, and the workgroup's scratch.
 struct Shared {
     uint32_t shift[NR_BG1_ENTRIES];
     uint16_t col[NR_BG1_ENTRIES];
     uint16_t row_start[48];
-    uint32_t synd[NR_BG1_ENTRIES];  // Z % 32 == 0: per edge, (word of its column, bit) in the packed decisions
+    // Z % 32 == 0: per row, max_degree entries (word of the edge's column, bit) in the packed
+    // decisions, the row's edges then entries of the zero column
+    uint32_t synd[kSyndSlots];
     uint32_t fails[kMaxGroup];  // per codeword of the task: the syndrome stamp of a failing check
     uint32_t task;
-    uint32_t vote[2];           // "some codeword still runs", alternate rounds
+    uint32_t vote[3];           // any_lane's flags, in turn
 };
 
-// Whether any lane of the workgroup has `pred` (one barrier). Two flags in turn: past the
-// barrier every lane has finished the round before, so the other flag can be cleared for
-// the round after, which the caller reaches only through further barriers.
+// Whether any lane of the workgroup has `pred` (one barrier). Three flags in turn: call k
+// sets flag k mod 3 and, past its barrier, clears flag k + 2 mod 3, which call k + 2 sets
+// only after the barrier of call k + 1; so calls may follow each other with no barrier
+// between them.
 __device__ __forceinline__ bool any_lane(Shared& sh, bool pred, uint32_t& round) {
-    const uint32_t slot = round & 1u;
+    const uint32_t slot = round % 3u;
     ++round;
     if (pred) sh.vote[slot] = 1u;
     __syncthreads();
-    if (threadIdx.x == 0) sh.vote[slot ^ 1u] = 0u;
+    if (threadIdx.x == 0) sh.vote[(slot + 2u) % 3u] = 0u;
     return sh.vote[slot] != 0u;
 }
 static_assert(sizeof(Shared) <= kSharedBytes, "the tables outgrow their LDS slot");
 
-// The signs of 32 consecutive posteriors (16-byte aligned) as one word, bit k = posterior k:
-// each funnel shift moves the word up one bit and brings in a sign bit at bit 0.
+// The packed decisions: per codeword and base column j, the signs of the column's Z
+// posteriors as Z/32 words (bit k of word w = posterior 32 w + k), stored twice in a row so
+// that a funnel shift across two words reads any 32 consecutive positions of the circulant;
+// after the last column a zero column (2 Z/32 zero words), where the padding entries of the
+// row table point.
+//
+// A lane packs one word from its 16-byte pieces with funnel shifts, each shift bringing in
+// one sign bit. Words are 128 (FP32) or 64 (FP16) bytes apart, so lanes reading piece k
+// together would hit the same LDS banks; each lane starts at its own piece instead (lanes
+// 2m, 2m+1 at piece m in FP32, lanes 4m..4m+3 at piece m in FP16), which puts the word
+// together rotated by that many pieces, and one funnel shift turns it back.
 template <bool HALF>
-__device__ __forceinline__ uint32_t pack_signs(const typename Store<HALF>::T* p) {
-    const uint4* v = reinterpret_cast<const uint4*>(p);
-    uint32_t acc = 0;
+__device__ __forceinline__ uint32_t piece_signs(uint32_t acc, uint4 x) {
     if constexpr (!HALF) {
-#pragma unroll
-        for (int k = 7; k >= 0; --k) {
-            const uint4 x = v[k];
-            acc = __builtin_amdgcn_alignbit(acc, x.w, 31u);
-            acc = __builtin_amdgcn_alignbit(acc, x.z, 31u);
-            acc = __builtin_amdgcn_alignbit(acc, x.y, 31u);
-            acc = __builtin_amdgcn_alignbit(acc, x.x, 31u);
-        }
+        acc = __builtin_amdgcn_alignbit(acc, x.w, 31u);
+        acc = __builtin_amdgcn_alignbit(acc, x.z, 31u);
+        acc = __builtin_amdgcn_alignbit(acc, x.y, 31u);
+        acc = __builtin_amdgcn_alignbit(acc, x.x, 31u);
     } else {
         // two halves a word: the high one's sign is bit 31, the low one's bit 15
+        const uint32_t d[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
-        for (int k = 3; k >= 0; --k) {
-            const uint4 x = v[k];
-            const uint32_t d[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-            for (int i = 3; i >= 0; --i) {
-                acc = __builtin_amdgcn_alignbit(acc, d[i], 31u);
-                acc = __builtin_amdgcn_alignbit(acc, d[i] << 16, 31u);
-            }
+        for (int i = 3; i >= 0; --i) {
+            acc = __builtin_amdgcn_alignbit(acc, d[i], 31u);
+            acc = __builtin_amdgcn_alignbit(acc, d[i] << 16, 31u);
         }
     }
     return acc;
+}
+
+template <bool HALF>
+__device__ __forceinline__ uint32_t pack_word(const typename Store<HALF>::T* p) {
+    constexpr int pieces = HALF ? 4 : 8, bits = 32 / pieces;
+    const uint32_t rot = (threadIdx.x >> (HALF ? 2 : 1)) & uint32_t(pieces - 1);
+    const char* w = reinterpret_cast<const char*>(p);
+    uint32_t acc = 0;
+    // pieces rot + pieces - 1, ..., rot + 1, rot (mod pieces): piece rot ends at bits 0..
+#pragma unroll
+    for (int k = pieces - 1; k >= 0; --k)
+        acc = piece_signs<HALF>(acc, *reinterpret_cast<const uint4*>(w + ((rot + uint32_t(k)) & uint32_t(pieces - 1)) * 16u));
+    // bit b of acc is bit (b + rot bits) mod 32 of the word: rotate left by rot bits (the
+    // funnel shift takes its amount mod 32)
+    return __builtin_amdgcn_alignbit(acc, acc, 32u - rot * uint32_t(bits));
 }
 
 // Whether the syndrome of the lane's codeword fails (ldpc.rs:555-557). Every codeword of
@@ -388,28 +451,33 @@ __device__ __forceinline__ bool syndrome_fails(const typename Store<HALF>::T* po
                                uint32_t stamp, int z, int n, int cnt, int g, int r, bool lane_ok) {
     using S = Store<HALF>;
     if ((z & 31) == 0) {
-        // packed: every lane packs the signs of 32 consecutive posteriors (one word of a
-        // column) with funnel shifts, bit k = posterior k; each column's words doubled
-        const int w32 = z >> 5, words = n >> 5, per_cw = Graph<BG>::cols * 2 * w32;
+        const int w32 = z >> 5, words = n >> 5, per_cw = (Graph<BG>::cols + 1) * 2 * w32;
         for (int f = threadIdx.x; f < cnt * words; f += blockDim.x) {
             const int gg = f / words, fw = f - gg * words;
             const int j = fw / w32, w = fw - j * w32;
-            const uint32_t word = pack_signs<HALF>(post_all + gg * n + fw * 32);
+            const uint32_t word = pack_word<HALF>(post_all + gg * n + fw * 32);
             uint32_t* hd = hd_all + gg * per_cw + j * 2 * w32;
             hd[w] = word;
             hd[w32 + w] = word;
         }
         __syncthreads();
+        // a row word of 32 parities: the XOR of the funnel-shifted column words of the row's
+        // edges, every row read through max_degree table entries so the reads go out together
+        constexpr int maxd = max_degree<BG>();
         const int per_task = Graph<BG>::rows * w32;
         for (int task = threadIdx.x; task < cnt * per_task; task += blockDim.x) {
             const int gg = task / per_task, rem = task - gg * per_task;
             const int i = rem / w32, w = rem - i * w32;
             const uint32_t* hd = hd_all + gg * per_cw + w;
+            const uint32_t* tab = sh.synd + i * maxd;
+            uint32_t xs[maxd];
+#pragma unroll
+            for (int k = 0; k < maxd; ++k) xs[k] = tab[k];
             uint32_t par = 0;
-            for (int eb = sh.row_start[i]; eb < sh.row_start[i + 1]; ++eb) {
-                const uint32_t x = sh.synd[eb];
-                const uint32_t* q = hd + (x >> 5);
-                par ^= __builtin_amdgcn_alignbit(q[1], q[0], x & 31u);
+#pragma unroll
+            for (int k = 0; k < maxd; ++k) {
+                const uint32_t* q = hd + (xs[k] >> 5);
+                par ^= __builtin_amdgcn_alignbit(q[1], q[0], xs[k] & 31u);
             }
             if (par) sh.fails[gg] = stamp;
         }
@@ -431,7 +499,60 @@ __device__ __forceinline__ bool syndrome_fails(const typename Store<HALF>::T* po
     return lane_ok && sh.fails[g] == stamp;
 }
 
-// full_llr (nr/ldpc.rs:191-197) of the lane's codeword: repeats summed in output order,
+// A short exact test ahead of the full syndrome: whether a check of a few chosen layers
+// fails on the lane's row (read directly, as the layer reads it). A failing check is
+// enough to keep decoding; the full syndrome runs only when these all hold. The layers
+// are the ones whose checks most often still fail on a codeword that does not yet satisfy
+// every check (measured on the reference decoder's iterations: about 97 % of such
+// iterations for base graph 1, 95 % for base graph 2, with the two layers below).
+template <int BG>
+struct Precheck;
+template <>
+struct Precheck<1> {
+    static constexpr int layers[2] = {11, 42};
+};
+template <>
+struct Precheck<2> {
+    static constexpr int layers[2] = {1, 26};
+};
+
+template <int BG, bool HALF, int ZC, int L>
+__device__ __forceinline__ uint32_t row_parity(const typename Store<HALF>::T* post, int r, int z,
+                                               const uint32_t* shift) {
+    using S = Store<HALF>;
+    constexpr int e0 = row_begin<BG>(L), deg = row_begin<BG>(L + 1) - e0;
+    constexpr uint32_t es = sizeof(typename S::T);
+    uint32_t par = 0;
+#pragma unroll
+    for (int q = 0; q < deg; ++q) {
+        const int e = e0 + q;
+        if constexpr (ZC != 0) {
+            const uint32_t s_q = shift_of<BG, ZC>(e) * es;
+            const uint32_t t = uint32_t(r) * es + s_q;
+            const uint32_t rr = s_q == 0 ? t : min(t, t - uint32_t(ZC) * es);
+            par ^= S::sign_at(rr + uint32_t(Graph<BG>::t[e].j) * ZC * es);
+        } else {
+            const uint32_t t = uint32_t(r) + shift[e];
+            par ^= S::sign(post, int(Graph<BG>::t[e].j) * z + int(min(t, t - uint32_t(z))));
+        }
+    }
+    return par;
+}
+
+// Whether the syndrome of a task of one codeword fails: the short test, then (if no check of
+// it fails) the full syndrome. The same answer on every lane.
+template <int BG, bool HALF, int ZC>
+__device__ __forceinline__ bool solo_syndrome_fails(const typename Store<HALF>::T* post_all, uint32_t* hd_all,
+                                                    Shared& sh, uint32_t stamp, uint32_t& round, int z, int n,
+                                                    int r, bool lane_ok) {
+    const uint32_t bad = lane_ok ? row_parity<BG, HALF, ZC, Precheck<BG>::layers[0]>(post_all, r, z, sh.shift) |
+                                       row_parity<BG, HALF, ZC, Precheck<BG>::layers[1]>(post_all, r, z, sh.shift)
+                                 : 0u;
+    if (any_lane(sh, bad != 0u, round)) return true;
+    syndrome_fails<BG, HALF>(post_all, hd_all, sh, stamp, z, n, 1, 0, r, lane_ok);
+    return sh.fails[0] == stamp;
+}
+
 // 0.0 on unsent bits, 30.0 on fillers; lane r writes positions r, r + z, ...
 template <bool HALF>
 __device__ __forceinline__ void place_llrs(typename Store<HALF>::T* post, const float* llr, const BatchClass& c,
@@ -465,14 +586,28 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
     const int limit = a.max_it ? int(a.max_it[id]) : c.max_iterations;
 
     if (lane_ok) place_llrs<HALF>(post, llr, c, r);
+    if ((z & 31) == 0) {
+        // the zero column of every codeword's packed decisions
+        const int w32 = z >> 5, per_cw = (Graph<BG>::cols + 1) * 2 * w32;
+        for (int t = tid; t < cnt * 2 * w32; t += blockDim.x) {
+            const int gg = t / (2 * w32);
+            hd_all[gg * per_cw + Graph<BG>::cols * 2 * w32 + (t - gg * 2 * w32)] = 0u;
+        }
+    }
     __syncthreads();
 
     Messages<Graph<BG>::rows, HALF> msg;
     msg.zero();
     bool finished = !lane_ok, restarted = false;
     uint32_t its = 0, ok = 0;
-    ++stamp;
-    if (!syndrome_fails<BG, HALF>(post_all, hd_all, sh, stamp, z, n, cnt, g, r, lane_ok) && lane_ok) {
+    // the syndrome: the short test first when the task has one codeword
+    auto fails_now = [&]() -> bool {
+        ++stamp;
+        if constexpr (SOLO) return solo_syndrome_fails<BG, HALF, ZC>(post_all, hd_all, sh, stamp, round, z, n, r,
+                                                                     lane_ok);
+        else return syndrome_fails<BG, HALF>(post_all, hd_all, sh, stamp, z, n, cnt, g, r, lane_ok);
+    };
+    if (!fails_now() && lane_ok) {
         finished = true;  // the channel LLRs are a codeword: 0 iterations
         ok = 1;
     }
@@ -504,8 +639,7 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
         const float* ab = learned ? a.params + size_t(it - 1) * 2 * Graph<BG>::rows : nullptr;
         sweep<BG, HALF, SOLO, ZC, 0>(post_all, r, uint32_t(g * n), z, !finished, sh.shift, msg, ab, c.a, c.b);
         __syncthreads();
-        ++stamp;
-        const bool fails = syndrome_fails<BG, HALF>(post_all, hd_all, sh, stamp, z, n, cnt, g, r, lane_ok);
+        const bool fails = fails_now();
         if (!finished && !fails) {
             finished = true;
             its = uint32_t(offset + it);
@@ -568,7 +702,7 @@ __global__ void __launch_bounds__(kMaxThreads, HALF ? 3 : 1) decode_layered_kern
     extern __shared__ __align__(16) uint32_t lds_dyn[];
     Shared& sh = *reinterpret_cast<Shared*>(reinterpret_cast<char*>(lds_dyn) + a.shared_off);
     for (int i = threadIdx.x; i < kMaxGroup; i += blockDim.x) sh.fails[i] = 0u;
-    if (threadIdx.x < 2) sh.vote[threadIdx.x] = 0u;
+    if (threadIdx.x < 3) sh.vote[threadIdx.x] = 0u;
     uint32_t stamp = 0, round = 0;
     int loaded = -1;
     uint32_t t = blockIdx.x;
@@ -600,15 +734,26 @@ __global__ void __launch_bounds__(kMaxThreads, HALF ? 3 : 1) decode_layered_kern
             // the shifts (read with uniform addresses), columns and row starts in LDS
             const int edges = c.bg == 1 ? Graph<1>::edges : Graph<2>::edges;
             const int rows = c.bg == 1 ? Graph<1>::rows : Graph<2>::rows;
-            const int w32 = c.z >> 5;
+            const int cols = c.bg == 1 ? Graph<1>::cols : Graph<2>::cols;
+            constexpr int maxd1 = max_degree<1>(), maxd2 = max_degree<2>();
+            const int maxd = c.bg == 1 ? maxd1 : maxd2;
+            const uint32_t w32 = uint32_t(c.z >> 5);
             for (int e = threadIdx.x; e < edges; e += blockDim.x) {
-                const uint32_t v = c.shift32[e], j = c.col[e];
-                sh.shift[e] = v;
-                sh.col[e] = uint16_t(j);
-                // the funnel shift of the packed syndrome: word j 2 w32 + v / 32, bit v % 32
-                sh.synd[e] = ((j * 2u * uint32_t(w32) + (v >> 5)) << 5) | (v & 31u);
+                sh.shift[e] = c.shift32[e];
+                sh.col[e] = uint16_t(c.col[e]);
             }
             for (int i = threadIdx.x; i <= rows; i += blockDim.x) sh.row_start[i] = c.row_start[i];
+            // the funnel shifts of the packed syndrome, row by row: edge (j, v) reads word
+            // j 2 w32 + v / 32 from bit v % 32; the padding reads the zero column
+            for (int s = threadIdx.x; s < rows * maxd; s += blockDim.x) {
+                const int i = s / maxd, k = s - i * maxd, e = int(c.row_start[i]) + k;
+                uint32_t x = (uint32_t(cols) * 2u * w32) << 5;
+                if (e < int(c.row_start[i + 1])) {
+                    const uint32_t v = c.shift32[e], j = c.col[e];
+                    x = ((j * 2u * w32 + (v >> 5)) << 5) | (v & 31u);
+                }
+                sh.synd[s] = x;
+            }
             __syncthreads();
             loaded = cls;
         }

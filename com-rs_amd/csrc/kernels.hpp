@@ -44,7 +44,7 @@ struct GenericArgs {
 // The layered min-sum kernel (messages in registers, posteriors in LDS).
 constexpr int kMaxThreads = 384;               // lanes of a workgroup: one per row of a layer
 constexpr int kMaxGroup = kMaxThreads / 2;     // codewords of one task (the smallest Z is 2)
-constexpr size_t kSharedBytes = 4096;          // a workgroup's tables and flags, after its posteriors
+constexpr size_t kSharedBytes = 8192;          // a workgroup's tables and flags, after its posteriors
 
 // One code of a decode batch, as the layered kernel sees it.
 struct BatchClass {
