@@ -76,7 +76,24 @@ void run_layered(const cf_code* h, int device, const LayeredRun& r, hipStream_t 
     a.ok = r.ok;
     const size_t off = layered_shared_offset(layered_lds_bytes(c.bg, int(c.z), g, r.half));
     a.shared_off = int32_t(off);
-    LayeredLaunch l{c.bg, r.half, g == 1, c.z == 384 && g == 1 ? 384 : 0, threads, int(a.n_tasks), off + kSharedBytes};
+    // persistent: as many workgroups as fit on the device, taking tasks from a counter, so
+    // that each loads the code's tables once and the codewords that stop early free their
+    // workgroup for the next
+    const bool solo = g == 1;
+    const int zc = c.z == 384 && solo ? 384 : 0;
+    const size_t lds = off + kSharedBytes;
+    int per_cu = 0, cus = 0;
+    check_hip(layered_occupancy(c.bg, r.half, solo, zc, threads, lds, &per_cu), "occupancy");
+    check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "device attribute");
+    const int grid = int(std::min<int64_t>(a.n_tasks, int64_t(std::max(per_cu, 1)) * std::max(cus, 1)));
+    LayeredLaunch l{c.bg, r.half, solo, zc, threads, std::max(grid, 1), lds};
+    // the counter lives in the stream's workspace, held until both launches are queued
+    WorkspaceSlot& slot = workspace(device, stream, kQueueSlot);
+    std::lock_guard<std::mutex> lock(slot.mu);
+    uint64_t* counter = static_cast<uint64_t*>(slot.ws.get(sizeof(uint64_t)));
+    if (!counter) fail(Kind::Device, "task counter: out of device memory");
+    check_hip(launch_zero(counter, 1, stream), "zero launch");
+    a.queue = reinterpret_cast<uint32_t*>(counter);
     check_hip(launch_layered(a, l, stream, true), "layered decode launch");
 }
 

@@ -52,6 +52,9 @@ struct WorkspaceSlot {
     Workspace ws;
 };
 WorkspaceSlot& workspace(int device, hipStream_t s, int purpose);
+// workspace purposes: 0 the general kernel, 1 training (learned_api.cpp), 2 the layered
+// kernel's task counter
+enum { kQueueSlot = 2 };
 DevCode dev_code(const NrCode& c, int device);
 // the generator's arguments for a code, a modulation and one SNR (campaign.cpp)
 TrialGenArgs make_generator_args(const NrCode& c, int device, int modulation, int snr_kind, double snr_db,

@@ -553,15 +553,38 @@ __device__ __forceinline__ bool solo_syndrome_fails(const typename Store<HALF>::
     return sh.fails[0] == stamp;
 }
 
-// 0.0 on unsent bits, 30.0 on fillers; lane r writes positions r, r + z, ...
-template <bool HALF>
+// full_llr (nr/ldpc.rs:191-197): position p is 0.0 plus its received LLRs in order (none
+// when p is not sent, several when E repeats it), or 30.0 on a filler. Lane r fills
+// positions r, r + z, ..., one per base column, a quarter of the columns at a time so that
+// the table and LLR loads of those positions go out together.
+template <int BG, bool HALF>
 __device__ __forceinline__ void place_llrs(typename Store<HALF>::T* post, const float* llr, const BatchClass& c,
-                                           int r) {
-    for (int p = r; p < c.n; p += c.z) {
-        float v = 0.0f;
-        for (uint32_t s = c.src_start[p]; s < c.src_start[p + 1]; ++s) v += llr[c.src[s]];
-        if (p >= c.k_info && p < c.k) v = 30.0f;
-        Store<HALF>::st(post, p, v);
+                                           int r, int z) {
+    constexpr int C = Graph<BG>::cols, U = (C + 3) / 4;
+#pragma unroll
+    for (int k0 = 0; k0 < C; k0 += U) {
+        uint32_t a[U], b[U], s0[U];
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (k0 + u >= C) continue;
+            const int p = r + (k0 + u) * z;
+            a[u] = c.src_start[p];
+            b[u] = c.src_start[p + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (k0 + u < C) s0[u] = c.src[a[u] < b[u] ? a[u] : 0u];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (k0 + u < C) v[u] = a[u] < b[u] ? 0.0f + llr[s0[u]] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (k0 + u >= C) continue;
+            const int p = r + (k0 + u) * z;
+            for (uint32_t s = a[u] + 1; s < b[u]; ++s) v[u] += llr[c.src[s]];
+            Store<HALF>::st(post, p, p >= c.k_info && p < c.k ? 30.0f : v[u]);
+        }
     }
 }
 
@@ -585,7 +608,7 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
     const float* llr = a.llr + (a.llr_off ? a.llr_off[id] : uint64_t(id) * uint64_t(c.e));
     const int limit = a.max_it ? int(a.max_it[id]) : c.max_iterations;
 
-    if (lane_ok) place_llrs<HALF>(post, llr, c, r);
+    if (lane_ok) place_llrs<BG, HALF>(post, llr, c, r, z);
     if ((z & 31) == 0) {
         // the zero column of every codeword's packed decisions
         const int w32 = z >> 5, per_cw = (Graph<BG>::cols + 1) * 2 * w32;
@@ -619,7 +642,7 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
             if (!finished) {
                 if (a.fallback) {
                     restarted = true;
-                    place_llrs<HALF>(post, llr, c, r);
+                    place_llrs<BG, HALF>(post, llr, c, r, z);
                     msg.zero();
                 } else {
                     finished = true;
