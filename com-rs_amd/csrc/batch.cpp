@@ -41,6 +41,8 @@ BatchClass batch_class(const NrCode& c, const DecoderSettings& s, int device, in
     k.row_start = t.row_start;
     k.src_start = t.src_start;
     k.src = t.src;
+    k.map = t.map;
+    k.direct = c.repeats ? 0 : 1;
     return k;
 }
 

@@ -561,6 +561,9 @@ template <int BG, bool HALF>
 __device__ __forceinline__ void place_llrs(typename Store<HALF>::T* post, const float* llr, const BatchClass& c,
                                            int r, int z) {
     constexpr int C = Graph<BG>::cols, U = (C + 3) / 4;
+    // opaque, or the table addresses of a launch of one code (loop-invariant) would be
+    // hoisted out of the task loop and held in registers through every sweep
+    asm volatile("" : "+v"(r));
 #pragma unroll
     for (int k0 = 0; k0 < C; k0 += U) {
         uint32_t a[U], b[U], s0[U];
@@ -588,6 +591,59 @@ __device__ __forceinline__ void place_llrs(typename Store<HALF>::T* post, const 
     }
 }
 
+// full_llr when no position is repeated: every position 0.0 (30.0 on a filler), then each
+// received LLR, as 0.0 plus it, at its position map[t]. The codeword's lanes load the LLRs
+// and the map as 16-byte vectors side by side, all before the first store. Every lane of
+// the workgroup calls it (one barrier).
+template <int BG, bool HALF>
+__device__ __forceinline__ void place_direct(typename Store<HALF>::T* post, const float* llr, const BatchClass& c,
+                                             int r, int z, bool lane_ok) {
+    using S = Store<HALF>;
+    constexpr int C = Graph<BG>::cols, V = (C + 3) / 4;
+    // opaque, or the filler tests of a launch of one code (loop-invariant) would be hoisted
+    // out of the task loop and held in registers through every sweep
+    asm volatile("" : "+v"(r));
+    if (lane_ok) {
+#pragma unroll
+        for (int k = 0; k < C; ++k) {
+            const int p = r + k * z;
+            S::st(post, p, p >= c.k_info && p < c.k ? 30.0f : 0.0f);
+        }
+    }
+    __syncthreads();
+    if (!lane_ok) return;
+    const int e = c.e;
+    if ((e & 3) == 0 && ((reinterpret_cast<uintptr_t>(llr) | reinterpret_cast<uintptr_t>(c.map)) & 15u) == 0) {
+        // at most V vectors a lane: e <= n - 2 z = (C - 2) z when nothing repeats
+        const int e4 = e >> 2;
+        const float4* l4 = reinterpret_cast<const float4*>(llr);
+        const uint4* m4 = reinterpret_cast<const uint4*>(c.map);
+        // in two halves, to stay within the registers
+        constexpr int H = (V + 1) / 2;
+#pragma unroll
+        for (int v0 = 0; v0 < V; v0 += H) {
+            float4 x[H];
+            uint4 m[H];
+#pragma unroll
+            for (int v = 0; v < H; ++v) {
+                const int i = min(r + (v0 + v) * z, e4 - 1);
+                x[v] = l4[i];
+                m[v] = m4[i];
+            }
+#pragma unroll
+            for (int v = 0; v < H; ++v) {
+                if (v0 + v >= V || r + (v0 + v) * z >= e4) continue;
+                S::st(post, int(m[v].x), 0.0f + x[v].x);
+                S::st(post, int(m[v].y), 0.0f + x[v].y);
+                S::st(post, int(m[v].z), 0.0f + x[v].z);
+                S::st(post, int(m[v].w), 0.0f + x[v].w);
+            }
+        }
+    } else {
+        for (int t = r; t < e; t += z) S::st(post, int(c.map[t]), 0.0f + llr[t]);
+    }
+}
+
 // One task: codewords order[first .. first + cnt) of class c (base graph BG). SOLO: the
 // task has one codeword (G = 1), whose posteriors start the LDS; ZC: Z, when every class of
 // the launch has the same (0: read at run time).
@@ -608,7 +664,8 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
     const float* llr = a.llr + (a.llr_off ? a.llr_off[id] : uint64_t(id) * uint64_t(c.e));
     const int limit = a.max_it ? int(a.max_it[id]) : c.max_iterations;
 
-    if (lane_ok) place_llrs<BG, HALF>(post, llr, c, r, z);
+    if (c.direct) place_direct<BG, HALF>(post, llr, c, r, z, lane_ok);
+    else if (lane_ok) place_llrs<BG, HALF>(post, llr, c, r, z);
     if ((z & 31) == 0) {
         // the zero column of every codeword's packed decisions
         const int w32 = z >> 5, per_cw = (Graph<BG>::cols + 1) * 2 * w32;
@@ -713,6 +770,8 @@ __device__ __forceinline__ BatchClass uni(const BatchClass& c) {
     u.row_start = uni(c.row_start);
     u.src_start = uni(c.src_start);
     u.src = uni(c.src);
+    u.map = uni(c.map);
+    u.direct = uni(c.direct);
     return u;
 }
 

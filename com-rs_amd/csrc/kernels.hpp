@@ -57,6 +57,8 @@ struct BatchClass {
     const uint16_t* row_start;  // rows_b + 1
     const uint32_t* src_start;  // n + 1: rate recovery (the output bits of every codeword bit)
     const uint32_t* src;        // e
+    const uint32_t* map;        // e: the codeword position of every output bit
+    int32_t direct;             // no position is repeated: the LLRs can be scattered by map
 };
 
 // A workgroup's unit of work: codewords order[first .. first + count) of class `cls`.

@@ -81,7 +81,7 @@ NrCode::NrCode(NrCode&& o) noexcept
       m(o.m), k_info(o.k_info), e(o.e), qm(o.qm), rv(o.rv), row_start(std::move(o.row_start)),
       col(std::move(o.col)), shift(std::move(o.shift)), col_start(std::move(o.col_start)),
       col_edge(std::move(o.col_edge)), map(std::move(o.map)), src_start(std::move(o.src_start)),
-      src(std::move(o.src)), core_inv(std::move(o.core_inv)), core_words(o.core_words), info(std::move(o.info)),
+      src(std::move(o.src)), repeats(o.repeats), core_inv(std::move(o.core_inv)), core_words(o.core_words), info(std::move(o.info)),
       sum_col(std::move(o.sum_col)), sum_start(std::move(o.sum_start)), sum_src(std::move(o.sum_src)), dec(o.dec),
       dev_(std::move(o.dev_)) {
     o.dev_.clear();
@@ -264,6 +264,8 @@ void NrCode::finish_tables() {
     std::vector<uint32_t> fill(src_start.begin(), src_start.end() - 1);
     src.assign(e, 0);
     for (uint32_t t = 0; t < e; ++t) src[fill[map[t]]++] = t;
+    repeats = false;
+    for (uint32_t p = 0; p < n; ++p) repeats = repeats || src_start[p + 1] - src_start[p] > 1;
 }
 
 uint32_t NrCode::max_row_degree() const {

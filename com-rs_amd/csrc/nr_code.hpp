@@ -52,6 +52,7 @@ struct NrCode {
     std::vector<uint32_t> map;        // e
     std::vector<uint32_t> src_start;  // n + 1
     std::vector<uint32_t> src;        // e
+    bool repeats = false;             // some codeword bit is carried by more than one output bit
     // encoder: row j of the inverse of the core block, as a bit mask over the 4Z checks
     std::vector<uint64_t> core_inv;   // 4Z rows x words
     uint32_t core_words = 0;
