@@ -640,7 +640,22 @@ __device__ __forceinline__ void place_direct(typename Store<HALF>::T* post, cons
             }
         }
     } else {
-        for (int t = r; t < e; t += z) S::st(post, int(c.map[t]), 0.0f + llr[t]);
+        // one LLR a load, a quarter of the lane's at a time (at most C - 2 of them)
+        constexpr int Q = (C + 3) / 4;
+#pragma unroll
+        for (int u0 = 0; u0 < C; u0 += Q) {
+            float x[Q];
+            uint32_t m[Q];
+#pragma unroll
+            for (int u = 0; u < Q; ++u) {
+                const int t = min(r + (u0 + u) * z, e - 1);
+                x[u] = llr[t];
+                m[u] = c.map[t];
+            }
+#pragma unroll
+            for (int u = 0; u < Q; ++u)
+                if (u0 + u < C && r + (u0 + u) * z < e) S::st(post, int(m[u]), 0.0f + x[u]);
+        }
     }
 }
 

@@ -341,9 +341,10 @@ const NrCode::DeviceTables& NrCode::device_tables(int device) const {
     if (dev_.size() <= size_t(device)) dev_.resize(device + 1);
     DeviceTables& t = dev_[device];
     if (t.base) return t;
-    // one buffer: u16 tables, then u32 tables (4-byte aligned)
+    // one buffer: u16 tables, then u32 tables from a 16-byte boundary, the map first (read as
+    // 16-byte vectors)
     size_t n16 = row_start.size() + col.size() + shift.size() + col_start.size() + col_edge.size();
-    n16 = (n16 + 1) & ~size_t(1);
+    n16 = (n16 + 7) & ~size_t(7);
     size_t n32 = src_start.size() + src.size() + map.size() + shift.size() + info.size();
     n32 = (n32 + 1) & ~size_t(1);
     size_t bytes = n16 * 2 + n32 * 4 + core_inv.size() * 8;
@@ -359,9 +360,9 @@ const NrCode::DeviceTables& NrCode::device_tables(int device) const {
     size_t o_rs = put16(row_start), o_col = put16(col), o_sh = put16(shift), o_cs = put16(col_start),
            o_ce = put16(col_edge);
     uint32_t* h32 = reinterpret_cast<uint32_t*>(host.data() + n16 * 2);
-    std::memcpy(h32, src_start.data(), src_start.size() * 4);
-    std::memcpy(h32 + src_start.size(), src.data(), src.size() * 4);
-    std::memcpy(h32 + src_start.size() + src.size(), map.data(), map.size() * 4);
+    std::memcpy(h32, map.data(), map.size() * 4);
+    std::memcpy(h32 + map.size(), src_start.data(), src_start.size() * 4);
+    std::memcpy(h32 + map.size() + src_start.size(), src.data(), src.size() * 4);
     uint32_t* h_shift = h32 + src_start.size() + src.size() + map.size();
     for (size_t i = 0; i < shift.size(); ++i) h_shift[i] = shift[i];
     std::memcpy(h_shift + shift.size(), info.data(), info.size() * 4);
@@ -386,9 +387,9 @@ const NrCode::DeviceTables& NrCode::device_tables(int device) const {
     t.shift = d16 + o_sh;
     t.col_start = d16 + o_cs;
     t.col_edge = d16 + o_ce;
-    t.src_start = d32;
-    t.src = d32 + src_start.size();
-    t.map = d32 + src_start.size() + src.size();
+    t.map = d32;
+    t.src_start = d32 + map.size();
+    t.src = d32 + map.size() + src_start.size();
     t.shift32 = d32 + src_start.size() + src.size() + map.size();
     t.info = info.empty() ? nullptr : t.shift32 + shift.size();
     t.core_inv = reinterpret_cast<const uint64_t*>(static_cast<const uint8_t*>(base) + n16 * 2 + n32 * 4);
