@@ -553,6 +553,25 @@ __device__ __forceinline__ bool solo_syndrome_fails(const typename Store<HALF>::
     return sh.fails[0] == stamp;
 }
 
+// The same for a task of several codewords: each lane tests its codeword's row, a failing
+// check marks the codeword, and the full syndrome runs only if a codeword still decoding
+// passed the test. `finished`: the lane's codeword has stopped (its answer is not used).
+template <int BG, bool HALF, int ZC>
+__device__ __forceinline__ bool group_syndrome_fails(const typename Store<HALF>::T* post_all, uint32_t* hd_all,
+                                                     Shared& sh, uint32_t stamp, uint32_t& round, int z, int n,
+                                                     int cnt, int g, int r, bool lane_ok, bool finished) {
+    const typename Store<HALF>::T* post = post_all + g * n;
+    const uint32_t bad = lane_ok ? row_parity<BG, HALF, ZC, Precheck<BG>::layers[0]>(post, r, z, sh.shift) |
+                                       row_parity<BG, HALF, ZC, Precheck<BG>::layers[1]>(post, r, z, sh.shift)
+                                 : 0u;
+    if (bad) sh.fails[g] = stamp;
+    __syncthreads();
+    const bool known = lane_ok && sh.fails[g] == stamp;
+    if (any_lane(sh, lane_ok && !finished && !known, round))
+        return syndrome_fails<BG, HALF>(post_all, hd_all, sh, stamp, z, n, cnt, g, r, lane_ok);
+    return known;
+}
+
 // full_llr (nr/ldpc.rs:191-197): position p is 0.0 plus its received LLRs in order (none
 // when p is not sent, several when E repeats it), or 30.0 on a filler. Lane r fills
 // positions r, r + z, ..., one per base column, a quarter of the columns at a time so that
@@ -695,12 +714,13 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
     msg.zero();
     bool finished = !lane_ok, restarted = false;
     uint32_t its = 0, ok = 0;
-    // the syndrome: the short test first when the task has one codeword
+    // the syndrome, the short test first
     auto fails_now = [&]() -> bool {
         ++stamp;
         if constexpr (SOLO) return solo_syndrome_fails<BG, HALF, ZC>(post_all, hd_all, sh, stamp, round, z, n, r,
                                                                      lane_ok);
-        else return syndrome_fails<BG, HALF>(post_all, hd_all, sh, stamp, z, n, cnt, g, r, lane_ok);
+        else return group_syndrome_fails<BG, HALF, ZC>(post_all, hd_all, sh, stamp, round, z, n, cnt, g, r, lane_ok,
+                                                       finished);
     };
     if (!fails_now() && lane_ok) {
         finished = true;  // the channel LLRs are a codeword: 0 iterations
