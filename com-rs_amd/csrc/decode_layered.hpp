@@ -270,19 +270,28 @@ __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post
     // t < z); with ZC, the LDS byte address (the posteriors start at 0) so that the column
     // base is the LDS instruction's immediate offset, and a zero shift (every extension
     // column) needs no wrap; otherwise the element index
-    auto address_of = [&](int e) -> int {
+    // With Z known only at run time the layer's shifts come from the LDS table, all of them
+    // before any posterior (the empty asm keeps the compiler from interleaving the two kinds
+    // of reads, which would cost an LDS round trip per pair of edges).
+    uint32_t sq[ZC != 0 ? 1 : deg];
+    if constexpr (ZC == 0) {
+#pragma unroll
+        for (int q = 0; q < deg; ++q) sq[q] = shift[e0 + q];
+        asm volatile("" ::: "memory");
+    }
+    auto address = [&](int q) -> int {
+        const int e = e0 + q;
         if constexpr (ZC != 0) {
             const uint32_t s_q = shift_of<BG, ZC>(e) * es;
             const uint32_t t = uint32_t(r) * es + s_q;
             const uint32_t rr = s_q == 0 ? t : min(t, t - uint32_t(ZC) * es);
             return int(rr) + int(Graph<BG>::t[e].j) * ZC * int(es);
         } else {
-            const uint32_t t = uint32_t(r) + shift[e];
+            const uint32_t t = uint32_t(r) + sq[ZC != 0 ? 0 : q];
             const uint32_t rr = min(t, t - uint32_t(z));
             return int(Graph<BG>::t[e].j) * z + int(rr) + int(base);
         }
     };
-    auto address = [&](int q) -> int { return address_of(e0 + q); };
     auto load = [&](int ad) -> float { return ZC != 0 ? S::ld_at(uint32_t(ad)) : S::ld(post, ad); };
     // FP16 kernels run three waves a SIMD (168 registers): their widest layers compute the
     // addresses twice rather than hold them

@@ -87,10 +87,11 @@ def test_f16_decisions_track_fp32():
     assert float((b32 == b16).all(1).float().mean()) > 0.98
 
 
-@pytest.mark.parametrize("bg,z", [(1, 2), (2, 16), (1, 24), (2, 32), (1, 36), (2, 352)])
+@pytest.mark.parametrize("bg,z", [(1, 2), (2, 16), (1, 24), (2, 32), (1, 36), (2, 352), (2, 384), (1, 384)])
 def test_codewords_packed_per_workgroup_bit_exact(bg, z):
     """floor(64k / Z) codewords per workgroup (Z = 2: 32 of them) decode as the oracle; with
-    one codeword per workgroup the lanes beyond Z idle (Z = 36: 28 of 64, Z = 352: 32 of 384)."""
+    one codeword per workgroup the lanes beyond Z idle (Z = 36: 28 of 64, Z = 352: 32 of 384);
+    Z = 384 runs the kernels with Z and the shifts compiled in, for both base graphs."""
     print(f"\n[packed workgroups] BG{bg} Z={z}", flush=True)
     code = F.NrLdpc.lifted(bg, z).set_decoder("normalized_min_sum", 0.75, True, 15)
     ref = O.Code.lifted(bg, z)
