@@ -24,9 +24,54 @@ __device__ inline T tanh_t(T x) { return tanh(x); }
 template <typename T>
 __device__ inline T atanh_t(T x) { return atanh(x); }
 
+// The reference's clamp of the tanh product, |p| <= 0.999999999999 (ldpc.rs:93, 465), as a
+// bound on the message: 2 atanh(0.999999999999).
+constexpr float kSpaCap = 28.324190418452805f;
+
+// phi(x) = -ln tanh(x/2) for x >= 0 (phi(phi(x)) = x), written as ln(1 + 2 e^-x / (1 - e^-x))
+// with 1 - e^-x = -expm1(-x) so that FP32 keeps its relative precision at both ends. The
+// ends are taken apart so that no zero or infinity reaches the library functions: below
+// 2^-12, tanh(x/2) = x/2 (1 - x^2/12 + ...) gives ln 2 - ln x within 5e-9 (and phi(0) = inf);
+// past 128 (also for x = inf), e^-x is 0 in FP32 and phi(x) = 0.
+__device__ inline float spa_phi(float x) {
+    if (x < 0x1p-12f) return x == 0.0f ? INFINITY : 0.693147180559945f - logf(x);
+    x = fminf(x, 128.0f);
+    return log1pf(2.0f * expf(-x) / -expm1f(-x));
+}
+
 // ldpc.rs:449-493 for one row of degree `deg`, messages in[] -> out[].
 template <typename T>
 __device__ inline void check_row(int rule, T alpha, const T (&in)[kMaxDeg], T (&out)[kMaxDeg], int deg) {
+    if constexpr (sizeof(T) == 4) {
+        if (rule == 0) {
+            // sum-product in FP32, in Gallager's form: |prod_j tanh(l_j/2)| = exp(-S) with
+            // S = sum_j phi(|l_j|), and 2 atanh(exp(-S)) = phi(S). The product form loses
+            // everything in FP32 once tanh(l/2) rounds to 1 (|l| > 17): the clamp then
+            // yields atanh(1) = inf. S over the others by prefix and suffix sums, the sign
+            // the parity of the others' sign bits (tanh keeps the sign of -0.0 too).
+            float s[kMaxDeg];
+            float prefix = 0.0f;
+            uint32_t parity = 0;
+#pragma unroll
+            for (int q = 0; q < kMaxDeg; ++q)
+                if (q < deg) {
+                    s[q] = spa_phi(fabsf(in[q]));
+                    out[q] = prefix;
+                    prefix += s[q];
+                    parity ^= __float_as_uint(in[q]);
+                }
+            float suffix = 0.0f;
+#pragma unroll
+            for (int q = kMaxDeg - 1; q >= 0; --q)
+                if (q < deg) {
+                    float m = fminf(spa_phi(out[q] + suffix), kSpaCap);
+                    suffix += s[q];
+                    out[q] = __uint_as_float(__float_as_uint(m) |
+                                             ((parity ^ __float_as_uint(in[q])) & 0x80000000u));
+                }
+            return;
+        }
+    }
     if (rule == 0) {
         // sum-product: exclusive prefix x suffix products of tanh(l/2), 2 atanh(clamp)
         T t[kMaxDeg];

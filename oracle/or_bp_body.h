@@ -2,12 +2,45 @@
  * Oracle (test infrastructure only): the body of the belief-propagation decoder of
  * crates/com-fec/src/ldpc.rs:449-557, written once for a message type REAL and included
  * by or_code.c twice: REAL = double (the reference's arithmetic) and REAL = float (the
- * same operations rounded to FP32, the exact target of the FP32 device kernels).
+ * same operations rounded to FP32, the exact target of the FP32 device kernels; with
+ * SPA_GALLAGER defined, the FP32 sum-product check in the log form the FP32 kernel uses).
  * Expects REAL, SFX(name) (name with a type suffix), FABS, FMAX, TANH, ATANH.
  */
 
 /* ldpc.rs:449-493 */
 static void SFX(check_message)(const or_dec* d, const REAL* in, int deg, REAL* out, REAL* scratch) {
+#ifdef SPA_GALLAGER
+    if (d->rule == OR_SPA) {
+        /* FP32: the same function as a sum of phi(x) = -ln tanh(x/2) = ln(1 + 2e^-x / (1 - e^-x)),
+           |prod_j tanh(l_j/2)| = exp(-S), 2 atanh(exp(-S)) = phi(S), and the reference's clamp
+           |p| <= 0.999999999999 as |out| <= 2 atanh(0.999999999999); the sign is the parity of
+           the others' sign bits. The product form rounds tanh(l/2) to 1 in FP32 for |l| > 17. */
+        const float cap = 28.324190418452805f;
+        /* phi with its ends taken apart, as the FP32 kernel computes it: below 2^-12
+           ln 2 - ln x (phi(0) = inf), the argument capped at 128 (phi = 0 past it) */
+        float prefix = 0.0f;
+        uint32_t parity = 0, bits;
+        for (int i = 0; i < deg; ++i) {
+            scratch[i] = or_spa_phi(fabsf(in[i]));
+            out[i] = prefix;
+            prefix += scratch[i];
+            memcpy(&bits, &in[i], 4);
+            parity ^= bits;
+        }
+        float suffix = 0.0f;
+        for (int i = deg - 1; i >= 0; --i) {
+            float sum = out[i] + suffix;
+            float m = fminf(or_spa_phi(sum), cap);
+            suffix += scratch[i];
+            uint32_t mb, ib;
+            memcpy(&mb, &m, 4);
+            memcpy(&ib, &in[i], 4);
+            mb |= (parity ^ ib) & 0x80000000u;
+            memcpy(&out[i], &mb, 4);
+        }
+        return;
+    }
+#endif
     if (d->rule == OR_SPA) {
         for (int i = 0; i < deg; ++i) scratch[i] = TANH(in[i] / (REAL)2.0);
         REAL prefix = 1.0;

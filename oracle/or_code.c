@@ -288,13 +288,24 @@ int or_nr_full_llr(const or_code* c, const double* llr, double* full) {
 #undef TANH
 #undef ATANH
 
+/* phi(x) = -ln tanh(x/2) = ln(1 + 2e^-x / (1 - e^-x)) for x >= 0, with its ends taken apart as
+   the FP32 device kernel takes them: below 2^-12, ln 2 - ln x (tanh(x/2) = x/2 (1 - x^2/12 + ...)),
+   phi(0) = inf; the argument capped at 128, where e^-x is 0 in FP32. */
+static float or_spa_phi(float x) {
+    if (x < 0x1p-12f) return x == 0.0f ? INFINITY : 0.693147180559945f - logf(x);
+    x = fminf(x, 128.0f);
+    return log1pf(2.0f * expf(-x) / -expm1f(-x));
+}
+
 #define REAL float
 #define SFX(name) name##_f32
 #define FABS fabsf
 #define FMAX fmaxf
 #define TANH tanhf
 #define ATANH atanhf
+#define SPA_GALLAGER
 #include "or_bp_body.h"
+#undef SPA_GALLAGER
 #undef REAL
 #undef SFX
 #undef FABS

@@ -128,6 +128,32 @@ def config2_llrs(esn0_db, count, k=8448, e=25344, qm=4, m=16):
     return ref, np.array(infos), np.array(llrs)
 
 
+@pytest.mark.parametrize("layered", [True, False], ids=["layered", "flooding"])
+def test_f32_sum_product_bg1_z384_16qam(layered):
+    """FP32 sum-product on BG1 Z=384, 16-QAM at Es/N0 4.2 dB, whose LLRs and posteriors run
+    far past the point where tanh(l/2) rounds to 1 in FP32: against the FP32 oracle (the
+    same log-domain check, device exp/expm1/log1p against glibc's) and the f64 reference
+    decoder, the same iterations, decisions and frame errors, posteriors within 1e-4
+    relative (1e-3 absolute near zero)."""
+    print(f"\n[f32 spa BG1 Z=384 16-QAM] layered={layered}", flush=True)
+    count, limit = 8, (20 if layered else 40)
+    ref, infos, llrs = config2_llrs(4.2, count)
+    llr32 = llrs.astype(np.float32)
+    full = np.array([ref.full_llr(l.astype(np.float64)) for l in llr32])
+    post64, it64 = ref.decode_many(full, rule=O.SPA, layered=layered, max_iterations=limit, threads=8)
+    post32, it32 = ref.decode_many_f32(full.astype(np.float32), rule=O.SPA, layered=layered,
+                                       max_iterations=limit, threads=8)
+    code = F.NrLdpc(8448, 25344, 1, 0, 4).set_decoder("sum_product", 0.0, layered, limit)
+    bits, its, post = code.decode_batch(cuda(llr32), want_post=True)
+    bits, its, post = bits.cpu().numpy(), its.cpu().numpy(), post.cpu().numpy().astype(np.float64)
+    assert not np.isnan(post).any() and np.abs(post64).max() > 60
+    assert np.array_equal(its, it64) and np.array_equal(its, it32) and (its < limit).all()
+    assert np.array_equal(bits, (post64[:, :8448] < 0).astype(np.uint8))
+    assert not (bits != infos).any()
+    np.testing.assert_allclose(post, post32, rtol=1e-4, atol=1e-3)
+    np.testing.assert_allclose(post, post64, rtol=1e-4, atol=1e-3)
+
+
 @pytest.mark.parametrize("rule", RULES, ids=[r[0] for r in RULES])
 @pytest.mark.parametrize("layered", [True, False], ids=["layered", "flooding"])
 def test_f32_bg2_z52_bit_exact_with_fp32_oracle(rule, layered):
