@@ -63,6 +63,8 @@ DevCode dev_code(const NrCode& c, int device) {
     d.src_start = t.src_start;
     d.src = t.src;
     d.info = t.info;
+    d.row_start32 = t.row_start32;
+    d.col32 = t.col32;
     return d;
 }
 

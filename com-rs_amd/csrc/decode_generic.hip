@@ -291,8 +291,128 @@ __global__ void __launch_bounds__(512) decode_generic_kernel(GenericArgs<T> a) {
     }
 }
 
+// Codes given by their rows (no lifting, Z = 1) under the layered schedule. The reference
+// updates the checks one after the other (ldpc.rs:522-531) and consecutive checks share
+// variables, so a codeword offers no parallelism inside a sweep. Each lane therefore
+// decodes codewords of its own, walking the rows in the reference's order; the lanes of a
+// wavefront walk the same row at once, and with the state laid out [index][lane] (the
+// posteriors, then the check-to-variable messages) each access of a wavefront reads or
+// writes 64 consecutive values. Lane g takes codewords g, g + lanes, ...; a codeword that
+// stops early frees its lane for the next one at once (the loop below is flat: a lane
+// loads, sweeps or finishes independently of the others).
+template <typename T>
+__global__ void __launch_bounds__(64) decode_rows_kernel(GenericArgs<T> a, int64_t lanes) {
+    const DevCode& c = a.code;
+    const int64_t gid = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= lanes) return;
+    const size_t L = size_t(lanes);
+    // the row tables as 32-bit words: every lane reads the same entry, so these are scalar loads
+    const uint32_t* __restrict__ row_start = c.row_start32;
+    const uint32_t* __restrict__ col = c.col32;
+    T* post = a.scratch + gid;                       // post[p * L], p < n
+    T* c2v = a.scratch + size_t(c.n) * L + gid;      // c2v[e * L], e < edges
+    // ldpc.rs:555-557: whether some check fails on the hard decisions
+    auto fails = [&]() -> bool {
+        for (int i = 0; i < c.rows_b; ++i) {
+            uint32_t par = 0;
+            for (uint32_t e = row_start[i]; e < row_start[i + 1]; ++e) par ^= post[size_t(col[e]) * L] < T(0);
+            if (par) return true;
+        }
+        return false;
+    };
+    auto finish = [&](int64_t w, int it) {
+        if (a.iterations) a.iterations[w] = uint32_t(it);
+        if (a.bits)
+            for (int i = 0; i < c.k_info; ++i)
+                a.bits[w * c.k_info + i] = post[size_t(c.info ? int(c.info[i]) : i) * L] < T(0);
+        if (a.post_out)
+            for (int p = 0; p < c.n; ++p) a.post_out[w * c.n + p] = post[size_t(p) * L];
+    };
+    int64_t w = gid;
+    int it = -1;  // sweeps done on codeword w; -1: not loaded
+    while (w < a.count) {
+        if (it < 0) {
+            // the channel LLRs of the full codeword, a priori LLRs on the information positions
+            for (int p = 0; p < c.n; ++p) {
+                T v;
+                if (a.input_full) {
+                    v = a.llr[w * c.n + p];
+                } else {
+                    v = T(0);
+                    for (uint32_t s = c.src_start[p]; s < c.src_start[p + 1]; ++s) v += a.llr[w * c.e + c.src[s]];
+                    if (p >= c.k_info && p < c.k) v = T(30);
+                }
+                post[size_t(p) * L] = v;
+            }
+            if (a.apriori)
+                for (int i = 0; i < c.k_info; ++i) {
+                    const int p = c.info ? int(c.info[i]) : i;
+                    post[size_t(p) * L] += a.apriori[w * c.k_info + i];
+                }
+            it = 0;
+            if (a.max_iterations == 0 || !fails()) {
+                finish(w, 0);
+                w += lanes;
+                it = -1;
+                continue;
+            }
+        }
+        // one sweep: ldpc.rs:522-531 for rows 0, 1, ..., m - 1 (no messages yet on the first)
+        for (int i = 0; i < c.rows_b; ++i) {
+            const uint32_t e0 = row_start[i];
+            const int deg = int(row_start[i + 1] - e0);
+            T in[kMaxDeg], out[kMaxDeg];
+            size_t at[kMaxDeg];
+#pragma unroll
+            for (int q = 0; q < kMaxDeg; ++q)
+                if (q < deg) {
+                    at[q] = size_t(col[e0 + q]) * L;
+                    const T old = it == 0 ? T(0) : c2v[size_t(e0 + q) * L];
+                    in[q] = post[at[q]] - old;
+                }
+            check_row<T>(a.rule, T(a.alpha), in, out, deg);
+#pragma unroll
+            for (int q = 0; q < kMaxDeg; ++q)
+                if (q < deg) {
+                    post[at[q]] = in[q] + out[q];
+                    c2v[size_t(e0 + q) * L] = out[q];
+                }
+        }
+        ++it;
+        if (it >= a.max_iterations || !fails()) {
+            finish(w, it);
+            w += lanes;
+            it = -1;
+        }
+    }
+}
+
+// Scratch of the lane-per-codeword kernel at most (its lanes are sized to fit).
+constexpr size_t kRowsScratchBytes = size_t(8) << 30;
+
+template <typename T>
+hipError_t launch_rows(const GenericArgs<T>& a0, Workspace& ws, hipStream_t stream) {
+    GenericArgs<T> a = a0;
+    const DevCode& c = a.code;
+    const size_t per_lane = (size_t(c.n) + size_t(c.edges_b)) * sizeof(T);
+    int64_t lanes = int64_t(kRowsScratchBytes / per_lane) & ~int64_t(63);
+    if (lanes < 64) lanes = 64;
+    if (lanes > a.count) lanes = a.count;
+    if (lanes < 1) return hipSuccess;
+    T* scratch = static_cast<T*>(ws.get(size_t(lanes) * per_lane));
+    if (!scratch) return hipErrorOutOfMemory;
+    a.scratch = scratch;
+    a.scratch_stride = 0;
+    // one wavefront per workgroup, so that few codewords still spread over many CUs
+    const int threads = 64;
+    const unsigned blocks = unsigned((lanes + threads - 1) / threads);
+    hipLaunchKernelGGL((decode_rows_kernel<T>), dim3(blocks), dim3(threads), 0, stream, a, lanes);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch(const GenericArgs<T>& a0, int device, hipStream_t stream, Workspace& ws) {
+    if (a0.layered && a0.code.z == 1) return launch_rows(a0, ws, stream);
     GenericArgs<T> a = a0;
     const DevCode& c = a.code;
     int threads = ((c.z + 63) / 64) * 64;

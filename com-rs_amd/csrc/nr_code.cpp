@@ -180,9 +180,12 @@ NrCode NrCode::from_rows(uint32_t n, uint32_t m, const uint32_t* row_start, cons
         if (p != rank)
             std::swap_ranges(h.begin() + size_t(p) * words, h.begin() + size_t(p + 1) * words,
                              h.begin() + size_t(rank) * words);
+        // the rows not yet taken as pivots are zero past colx (every column after it is a pivot,
+        // cleared from them, or was zero in all of them), so the pivot row ends at colx's word
+        const uint32_t used = colx / 64 + 1;
         for (uint32_t r = 0; r < m; ++r)
             if (r != rank && bit(r, colx))
-                for (uint32_t w = 0; w < words; ++w) h[size_t(r) * words + w] ^= h[size_t(rank) * words + w];
+                for (uint32_t w = 0; w < used; ++w) h[size_t(r) * words + w] ^= h[size_t(rank) * words + w];
         pivots.push_back(colx);
         ++rank;
     }
@@ -345,7 +348,8 @@ const NrCode::DeviceTables& NrCode::device_tables(int device) const {
     // 16-byte vectors)
     size_t n16 = row_start.size() + col.size() + shift.size() + col_start.size() + col_edge.size();
     n16 = (n16 + 7) & ~size_t(7);
-    size_t n32 = src_start.size() + src.size() + map.size() + shift.size() + info.size();
+    size_t n32 = src_start.size() + src.size() + map.size() + shift.size() + info.size() + row_start.size() +
+                 col.size();
     n32 = (n32 + 1) & ~size_t(1);
     size_t bytes = n16 * 2 + n32 * 4 + core_inv.size() * 8;
     std::vector<uint8_t> host(bytes, 0);
@@ -366,6 +370,10 @@ const NrCode::DeviceTables& NrCode::device_tables(int device) const {
     uint32_t* h_shift = h32 + src_start.size() + src.size() + map.size();
     for (size_t i = 0; i < shift.size(); ++i) h_shift[i] = shift[i];
     std::memcpy(h_shift + shift.size(), info.data(), info.size() * 4);
+    uint32_t* h_rs = h_shift + shift.size() + info.size();
+    for (size_t i = 0; i < row_start.size(); ++i) h_rs[i] = row_start[i];
+    uint32_t* h_col = h_rs + row_start.size();
+    for (size_t i = 0; i < col.size(); ++i) h_col[i] = col[i];
     uint64_t* h64 = reinterpret_cast<uint64_t*>(host.data() + n16 * 2 + n32 * 4);
     std::memcpy(h64, core_inv.data(), core_inv.size() * 8);
     int prev = 0;
@@ -392,6 +400,8 @@ const NrCode::DeviceTables& NrCode::device_tables(int device) const {
     t.src = d32 + map.size() + src_start.size();
     t.shift32 = d32 + src_start.size() + src.size() + map.size();
     t.info = info.empty() ? nullptr : t.shift32 + shift.size();
+    t.row_start32 = t.shift32 + shift.size() + info.size();
+    t.col32 = t.row_start32 + row_start.size();
     t.core_inv = reinterpret_cast<const uint64_t*>(static_cast<const uint8_t*>(base) + n16 * 2 + n32 * 4);
     return t;
 }

@@ -91,6 +91,8 @@ struct NrCode {
         const uint64_t* core_inv = nullptr;
         const uint32_t* shift32 = nullptr;  // the shifts as 32-bit words (scalar loads)
         const uint32_t* info = nullptr;     // K' information positions (codes given by rows), or null: 0..K'-1
+        const uint32_t* row_start32 = nullptr;  // row_start and col as 32-bit words (scalar loads)
+        const uint32_t* col32 = nullptr;
     };
     const DeviceTables& device_tables(int device) const;
     ~NrCode();

@@ -8,9 +8,14 @@ outputs are committed, and nothing at test time reads the reference.
   codewords for 28 (K', E), two messages each (u: info bits, c: transmitted bits).
 - aff3ct_*.json: the LDPC entries of validation/aff3ct/ (AFF3CT v4.7.1 FER/BER curves).
 - *.alist: the PEG matrices those AFF3CT runs decoded.
+- dvbs2_n16200_k7200.json: the parity accumulator address table of the DVB-S2 LDPC code
+  of the short FECFRAME, rate 1/2 (ETSI EN 302 307-1 V1.4.1, Annex C), as the reference
+  holds it (crates/com-systems/src/dvbs2/tables.rs, N16200_P9000): the code of the AFF3CT
+  run ldpc_dvbs2_short_r1_2_layered_spa_i50.
 """
 import json
 import os
+import re
 import shutil
 
 import numpy as np
@@ -31,6 +36,14 @@ def main():
             shutil.copy(os.path.join(REF, "validation/aff3ct", name), os.path.join(OUT, "aff3ct_" + name))
     for name in ("peg_n576_k288.alist", "peg_n648_k540.alist"):
         shutil.copy(os.path.join(REF, "validation/aff3ct/matrices", name), os.path.join(OUT, name))
+    text = open(os.path.join(REF, "crates/com-systems/src/dvbs2/tables.rs")).read()
+    body = re.search(r"const N16200_P9000: &\[u16\] = &\[(.*?)\];", text, re.S).group(1)
+    table = [int(x) for x in body.split(",") if x.strip()]
+    json.dump({"n": 16200, "k": 7200, "group": 360,
+               "source": "ETSI EN 302 307-1 V1.4.1 Annex C, short FECFRAME, rate 1/2 "
+                         "(crates/com-systems/src/dvbs2/tables.rs N16200_P9000)",
+               "layout": "per group of 360 information bits: the count of addresses, then the addresses",
+               "table": table}, open(os.path.join(OUT, "dvbs2_n16200_k7200.json"), "w"))
 
 
 if __name__ == "__main__":

@@ -50,3 +50,25 @@ def test_rows_too_long_for_the_device():
     with pytest.raises(F.ComfecError) as e:
         F.NrLdpc.from_rows(40, [list(range(20)), list(range(20, 40))])
     assert e.value.kind == "Config"
+
+
+def test_dvbs2_short_code_structure_and_encoder():
+    """The DVB-S2 short rate-1/2 code from the standard's table: the degree distribution of
+    the standard (1800 information bits of degree 8, 5400 of degree 3, the accumulator's
+    parity bits of degree 2 but the last of degree 1; checks of 4 to 7 bits), K = 7200 with
+    the information bits first (the accumulator is lower triangular, so the parity
+    columns are the pivots), and codewords of the host encoder satisfy H c = 0."""
+    from dvbs2 import dvbs2_short_r1_2_rows
+    n, k, rows = dvbs2_short_r1_2_rows()
+    deg = np.zeros(n, int)
+    for r in rows:
+        deg[r] += 1
+    assert (np.sum(deg[:k] == 8), np.sum(deg[:k] == 3)) == (1800, 5400)
+    assert np.all(deg[k:n - 1] == 2) and deg[n - 1] == 1
+    assert {len(r) for r in rows} == {4, 5, 6, 7} and sum(len(r) for r in rows) == 48599
+    code = F.NrLdpc.from_rows(n, rows)
+    assert code.k == k and code.n == n
+    ref = O.Code.from_rows(n, rows)
+    u = np.random.default_rng(5).integers(0, 2, k).astype(np.uint8)
+    cw = code.encode_block(u)
+    assert ref.is_codeword(cw) and np.array_equal(cw[:k], u)
