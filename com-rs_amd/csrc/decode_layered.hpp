@@ -206,10 +206,14 @@ struct Messages<ROWS, true> {
             pk[i] = 0u;
         }
     }
+    // The widened magnitudes are made opaque: the compiler would otherwise move the
+    // conversion past each edge's choice between them (one conversion per edge, not two
+    // per row).
     template <int I>
     __device__ __forceinline__ void get(float& a, float& b) const {
         a = h2f(gh[I] & 0xffffu);
         b = h2f(gh[I] >> 16);
+        asm volatile("" : "+v"(a), "+v"(b));
     }
     template <int I>
     __device__ __forceinline__ void put(float& a, float& b) {
@@ -217,6 +221,7 @@ struct Messages<ROWS, true> {
         gh[I] = ha | (hb << 16);
         a = h2f(ha);
         b = h2f(hb);
+        asm volatile("" : "+v"(a), "+v"(b));
     }
 };
 
