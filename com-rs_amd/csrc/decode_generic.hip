@@ -11,6 +11,8 @@
 // Workgroups are persistent: each walks codewords blockIdx.x, +gridDim.x, ...
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace comfec {
@@ -399,7 +401,12 @@ hipError_t launch_rows(const GenericArgs<T>& a0, Workspace& ws, hipStream_t stre
     if (lanes < 64) lanes = 64;
     if (lanes > a.count) lanes = a.count;
     if (lanes < 1) return hipSuccess;
+    // fewer lanes (each then decodes more codewords) when the device cannot hold them all
     T* scratch = static_cast<T*>(ws.get(size_t(lanes) * per_lane));
+    while (!scratch && lanes > 64) {
+        lanes = std::max<int64_t>(64, (lanes / 2) & ~int64_t(63));
+        scratch = static_cast<T*>(ws.get(size_t(lanes) * per_lane));
+    }
     if (!scratch) return hipErrorOutOfMemory;
     a.scratch = scratch;
     a.scratch_stride = 0;
