@@ -242,7 +242,8 @@ struct Messages<ROWS, true> {
 //    matter: three inputs at a time give their min3 and med3, and a pair (t1 <= t2) merges
 //    into (m1 <= m2) as min(m1, t1) and med3(m1, t1, min(m2, t2)).
 // The stored state keeps the output sign bits (edge q at bit deg-1-q, shifted in with a
-// funnel shift) and one index whose magnitude is g(m2).
+// funnel shift) and one index whose magnitude is g(m2); a message is rebuilt as that
+// magnitude with its sign bit replaced by the stored one, which is the output bit for bit.
 template <int BG, bool HALF, bool SOLO, int ZC, int I>
 __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post, int r, uint32_t gofs, int z,
                                       const uint32_t* __restrict__ shift, Messages<Graph<BG>::rows, HALF>& msg,
@@ -308,7 +309,9 @@ __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post
         const int ad = address(q);
         if constexpr (keep) addr[q] = ad;
         const float mag = uint32_t(q) == i1o ? g2o : g1o;
-        const float old = __uint_as_float(__float_as_uint(mag) ^ ((pko << (32 - deg + q)) & 0x80000000u));
+        // bit 31 from the stored word, the rest from the magnitude: (c ? a : b) bitwise
+        const float old = __uint_as_float(
+            __builtin_amdgcn_bitop3_b32(pko << (32 - deg + q), __float_as_uint(mag), 0x80000000u, 0xe4));
         v2c[q] = load(ad) - old;
     }
     // the row's sign product: three-input XORs
@@ -342,16 +345,22 @@ __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post
     float g1 = a * fmaxf(m1 - b, 0.0f);
     float g2 = a * fmaxf(m2 - b, 0.0f);
     msg.template put<I>(g1, g2);
+    // the magnitudes with the row's sign product applied once: an edge's output is then
+    // its choice XOR its own input's sign bit, and the stored bit is the output's sign
     parity &= 0x80000000u;
+    uint32_t g1s = __float_as_uint(g1) ^ parity, g2s = __float_as_uint(g2) ^ parity;
+    // opaque, or the compiler folds the sign product back into every edge
+    asm volatile("" : "+v"(g1s), "+v"(g2s));
     uint32_t signs = 0, i1 = 0;
 #pragma unroll
     for (int q = 0; q < deg; ++q) {
         const bool first = fabsf(v2c[q]) == m1;
-        const float mag = first ? g2 : g1;
+        const uint32_t sel = first ? g2s : g1s;
         i1 = first ? uint32_t(q) : i1;
-        const uint32_t sb = __float_as_uint(v2c[q]) ^ parity;
-        signs = __builtin_amdgcn_alignbit(signs, sb, 31u);
-        const float out = __uint_as_float(__float_as_uint(mag) ^ (sb & 0x80000000u));
+        // (a & c) ^ b bitwise
+        const uint32_t ob = __builtin_amdgcn_bitop3_b32(__float_as_uint(v2c[q]), sel, 0x80000000u, 0x6c);
+        signs = __builtin_amdgcn_alignbit(signs, ob, 31u);
+        const float out = __uint_as_float(ob);
         const int ad = keep ? addr[keep ? q : 0] : address(q);
         if constexpr (ZC != 0) S::st_at(uint32_t(ad), v2c[q] + out);
         else S::st(post, ad, v2c[q] + out);
