@@ -545,7 +545,8 @@ __device__ __forceinline__ uint32_t row_parity(const typename Store<HALF>::T* po
     using S = Store<HALF>;
     constexpr int e0 = row_begin<BG>(L), deg = row_begin<BG>(L + 1) - e0;
     constexpr uint32_t es = sizeof(typename S::T);
-    uint32_t par = 0;
+    // every sign first, then their parity, so that the reads go out together
+    uint32_t sg[deg];
 #pragma unroll
     for (int q = 0; q < deg; ++q) {
         const int e = e0 + q;
@@ -553,12 +554,15 @@ __device__ __forceinline__ uint32_t row_parity(const typename Store<HALF>::T* po
             const uint32_t s_q = shift_of<BG, ZC>(e) * es;
             const uint32_t t = uint32_t(r) * es + s_q;
             const uint32_t rr = s_q == 0 ? t : min(t, t - uint32_t(ZC) * es);
-            par ^= S::sign_at(rr + uint32_t(Graph<BG>::t[e].j) * ZC * es);
+            sg[q] = S::sign_at(rr + uint32_t(Graph<BG>::t[e].j) * ZC * es);
         } else {
             const uint32_t t = uint32_t(r) + shift[e];
-            par ^= S::sign(post, int(Graph<BG>::t[e].j) * z + int(min(t, t - uint32_t(z))));
+            sg[q] = S::sign(post, int(Graph<BG>::t[e].j) * z + int(min(t, t - uint32_t(z))));
         }
     }
+    uint32_t par = 0;
+#pragma unroll
+    for (int q = 0; q < deg; ++q) par ^= sg[q];
     return par;
 }
 
