@@ -1,7 +1,8 @@
 """A short driver for profilers and timings: decode one batch of the BG1 Z=384 16-QAM
 workload a few times with the plain or the learned decoder (FP32 or FP16 storage), or the
 uneven workload (codewords of every NR base graph and lifting size 16..384, BPSK/QPSK,
-Eb/N0 uniform in [0, 3] dB, iteration limits 5..25) through one batch plan, or copy a
+Eb/N0 uniform in [0, 3] dB, iteration limits 5..25) through one batch plan, or the DVB-S2
+short rate-1/2 code (a code given by its rows: the lane-per-codeword kernel), or copy a
 buffer of known size (the yardstick of the HBM byte counters). Prints the mean time of a
 step measured with HIP events.
 
@@ -9,6 +10,7 @@ step measured with HIP events.
     python com-rs_amd/profile_decode.py --decoder plain --half
     python com-rs_amd/profile_decode.py --uneven 20000 --half --graph
     python com-rs_amd/profile_decode.py --copy-mib 1024 --steps 5
+    python com-rs_amd/profile_decode.py --dvbs2 20000 --ebn0 0.9 --steps 2
 
 Under rocprofv3, e.g.:
     rocprofv3 --kernel-trace --stats -d OUT -o plain -- python3 com-rs_amd/profile_decode.py
@@ -34,11 +36,16 @@ def main():
     p.add_argument("--uneven", type=int, default=0, help="codewords of the uneven workload")
     p.add_argument("--graph", action="store_true", help="uneven: replay a captured HIP graph")
     p.add_argument("--tune", action="store_true", help="uneven: autotune the workgroup width first")
+    p.add_argument("--dvbs2", type=int, default=0,
+                   help="codewords of the DVB-S2 short rate-1/2 code (layered sum-product, FP32, 50 it)")
+    p.add_argument("--ebn0", type=float, default=0.9, help="dvbs2: Eb/N0 in dB")
     a = p.parse_args()
     import torch
     import comfec as F
     if a.uneven:
         return uneven(a, torch, F)
+    if a.dvbs2:
+        return dvbs2(a, torch, F)
     if a.copy_mib:
         src = torch.empty(a.copy_mib << 18, dtype=torch.float32, device="cuda").fill_(1.0)
         dst = torch.empty_like(src)
@@ -79,6 +86,48 @@ def timed(fn, steps, torch):
 
 LIFTS = [16, 18, 20, 22, 24, 26, 28, 30, 32, 36, 40, 44, 48, 52, 56, 60, 64, 72, 80, 88, 96, 104, 112, 120, 128,
          144, 160, 176, 192, 208, 224, 240, 256, 288, 320, 352, 384]
+
+
+def dvbs2(a, torch, F):
+    """The DVB-S2 LDPC code of the short FECFRAME, rate 1/2 (16200 bits, 9000 checks), built
+    from the standard's address table (tests/golden/dvbs2_n16200_k7200.json) by the rule of
+    crates/com-systems/src/dvbs2/fec.rs:213-243; BPSK over AWGN, the all-zero codeword, LLRs
+    from torch's generator; one batch decoded `steps` times by the lane-per-codeword kernel."""
+    import json
+    import numpy as np
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    fx = json.load(open(os.path.join(root, "tests", "golden", "dvbs2_n16200_k7200.json")))
+    n, k, group, table = fx["n"], fx["k"], fx["group"], fx["table"]
+    m, q = n - k, (n - k) // group
+    rows = [[] for _ in range(m)]
+    bit = i = 0
+    while i < len(table):
+        for l in range(group):
+            for x in table[i + 1:i + 1 + table[i]]:
+                rows[(x + l * q) % m].append(bit)
+            bit += 1
+        i += 1 + table[i]
+    for r in range(m):
+        rows[r] += ([k + r - 1] if r > 0 else []) + [k + r]
+    code = F.NrLdpc.from_rows(n, rows).set_decoder("sum_product", 0.0, True, 50)
+    s2 = 1.0 / (2.0 * (k / n) * 10.0 ** (a.ebn0 / 10.0))
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    noise = torch.randn((a.dvbs2, n), generator=g, dtype=torch.float32, device="cuda")
+    llr = ((1.0 + float(np.sqrt(s2)) * noise) * float(2.0 / s2)).contiguous()
+    del noise
+    code.decode_batch(llr)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(a.steps):
+        bits, its, _ = code.decode_batch(llr)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / a.steps
+    it = float(its.double().mean().item())
+    print(f"dvbs2 short 1/2: {a.dvbs2} codewords at {a.ebn0} dB, mean iterations {it:.3f}, "
+          f"max {int(its.max().item())}, {ms:.3f} ms per launch, {a.dvbs2 / ms * 1e3:.0f} codewords/s", flush=True)
 
 
 def uneven(a, torch, F):
