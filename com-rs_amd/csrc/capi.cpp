@@ -65,6 +65,7 @@ DevCode dev_code(const NrCode& c, int device) {
     d.info = t.info;
     d.row_start32 = t.row_start32;
     d.col32 = t.col32;
+    d.max_deg = int32_t(c.max_row_degree());
     return d;
 }
 

@@ -389,6 +389,201 @@ __global__ void __launch_bounds__(64) decode_rows_kernel(GenericArgs<T> a, int64
     }
 }
 
+// ---- codes given by their rows, a group of G lanes per codeword ------------------------
+//
+// The same schedule as decode_rows_kernel, with the edges of a check spread over a group of
+// G = 8 or 16 lanes (edge q on lane q of the group): G times as many codewords in flight for
+// a batch, and a check's work done side by side. The groups sit inside the 16-lane rows of
+// the wavefront, so their exchanges are DPP moves: the check's minimum, first minimum and
+// sign parity by butterflies (order-free: a minimum is a selection), the sum-product's
+// prefix and suffix by sequential shifts in the reference's order, so that every result is
+// the one check_row computes. State laid out [index][group].
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp32(uint32_t x) {
+    return uint32_t(__builtin_amdgcn_update_dpp(0, int(x), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp(float x) {
+    return __uint_as_float(dpp32<CTRL>(__float_as_uint(x)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double x) {
+    const uint64_t b = __double_as_longlong(x);
+    const uint64_t lo = dpp32<CTRL>(uint32_t(b)), hi = dpp32<CTRL>(uint32_t(b >> 32));
+    return __longlong_as_double(int64_t(lo | (hi << 32)));
+}
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp(uint32_t x) {
+    return dpp32<CTRL>(x);
+}
+
+constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kHalfMirror = 0x141, kRowMirror = 0x140;
+constexpr int kRowShr1 = 0x111, kRowShl1 = 0x101;
+
+// op over the G lanes of the group, the result in every lane of it
+template <int G, typename V, typename F>
+__device__ __forceinline__ V group_all(V x, F op) {
+    x = op(x, dpp<kQuadXor1>(x));
+    x = op(x, dpp<kQuadXor2>(x));
+    x = op(x, dpp<kHalfMirror>(x));
+    if constexpr (G == 16) x = op(x, dpp<kRowMirror>(x));
+    return x;
+}
+
+// exclusive scan in lane order: lane q gets ((id op v_0) op v_1) ... op v_{q-1}
+template <typename V, typename F>
+__device__ __forceinline__ V group_prefix(V v, V id, int q, int deg, F op) {
+    V pre = id;
+    for (int k = 1; k < deg; ++k) {
+        const V cand = dpp<kRowShr1>(op(pre, v));
+        pre = q == k ? cand : pre;
+    }
+    return pre;
+}
+// and from the end: lane q gets ((id op v_{deg-1}) op v_{deg-2}) ... op v_{q+1}
+template <typename V, typename F>
+__device__ __forceinline__ V group_suffix(V v, V id, int q, int deg, F op) {
+    V suf = id;
+    for (int k = deg - 2; k >= 0; --k) {
+        const V cand = dpp<kRowShl1>(op(suf, v));
+        suf = q == k ? cand : suf;
+    }
+    return suf;
+}
+
+// check_row for the edge of lane q (the lanes q >= deg take part with neutral values)
+template <int G, typename T>
+__device__ __forceinline__ T group_check(int rule, T alpha, T in, int q, int deg) {
+    const bool act = q < deg;
+    if (rule == 0) {
+        if constexpr (sizeof(T) == 4) {
+            const float s = act ? spa_phi(fabsf(in)) : 0.0f;
+            const float pre = group_prefix(s, 0.0f, q, deg, [](float x, float y) { return x + y; });
+            const float suf = group_suffix(s, 0.0f, q, deg, [](float x, float y) { return x + y; });
+            const uint32_t parity = group_all<G>(act ? __float_as_uint(in) : 0u,
+                                                 [](uint32_t x, uint32_t y) { return x ^ y; });
+            const float m = fminf(spa_phi(pre + suf), kSpaCap);
+            return __uint_as_float(__float_as_uint(m) | ((parity ^ __float_as_uint(in)) & 0x80000000u));
+        } else {
+            const T t = act ? tanh_t(in / T(2)) : T(1);
+            const T pre = group_prefix(t, T(1), q, deg, [](T x, T y) { return x * y; });
+            const T suf = group_suffix(t, T(1), q, deg, [](T x, T y) { return x * y; });
+            const T bound = T(0.999999999999);
+            T p = pre * suf;
+            p = p < -bound ? -bound : (p > bound ? bound : p);
+            return T(2) * atanh_t(p);
+        }
+    }
+    const T a = act ? T(fabs(in)) : T(INFINITY);
+    const T m1 = group_all<G>(a, [](T x, T y) { return x < y ? x : y; });
+    const uint32_t i1 = group_all<G>(act && a == m1 ? uint32_t(q) : uint32_t(G),
+                                     [](uint32_t x, uint32_t y) { return x < y ? x : y; });
+    const T m2 = group_all<G>(uint32_t(q) == i1 ? T(INFINITY) : a, [](T x, T y) { return x < y ? x : y; });
+    const uint32_t neg = group_all<G>(act && in < T(0) ? 1u : 0u, [](uint32_t x, uint32_t y) { return x ^ y; });
+    T g1, g2;
+    if (rule == 1) {
+        g1 = alpha * m1;
+        g2 = alpha * m2;
+    } else {
+        g1 = fmax(m1 - alpha, T(0));
+        g2 = fmax(m2 - alpha, T(0));
+    }
+    const T g = uint32_t(q) == i1 ? g2 : g1;
+    return (neg != 0) != (in < T(0)) ? -g : g;
+}
+
+template <typename T, int G>
+__global__ void __launch_bounds__(64) decode_groups_kernel(GenericArgs<T> a, int64_t slots) {
+    const DevCode& c = a.code;
+    const int64_t slot = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / G;
+    const int q = int(threadIdx.x) & (G - 1);
+    if (slot >= slots) return;  // whole groups: G divides the workgroup
+    const size_t L = size_t(slots);
+    const uint32_t* __restrict__ row_start = c.row_start32;
+    const uint32_t* __restrict__ col = c.col32;
+    T* post = a.scratch + slot;                      // post[p * L], p < n
+    T* c2v = a.scratch + size_t(c.n) * L + slot;     // c2v[e * L], e < edges
+    // a lane reads what another lane of its wavefront wrote: in order within a wavefront
+    auto sync = []() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); };
+    // ldpc.rs:555-557, the same answer in every lane of the group
+    auto fails = [&]() -> bool {
+        for (int i = 0; i < c.rows_b; ++i) {
+            const uint32_t e0 = row_start[i];
+            const int deg = int(row_start[i + 1] - e0);
+            uint32_t bit = 0;
+            if (q < deg) bit = post[size_t(col[e0 + q]) * L] < T(0);
+            if (group_all<G>(bit, [](uint32_t x, uint32_t y) { return x ^ y; })) return true;
+        }
+        return false;
+    };
+    auto finish = [&](int64_t w, int it) {
+        if (a.iterations && q == 0) a.iterations[w] = uint32_t(it);
+        if (a.bits)
+            for (int i = q; i < c.k_info; i += G)
+                a.bits[w * c.k_info + i] = post[size_t(c.info ? int(c.info[i]) : i) * L] < T(0);
+        if (a.post_out)
+            for (int p = q; p < c.n; p += G) a.post_out[w * c.n + p] = post[size_t(p) * L];
+    };
+    int64_t w = slot;
+    int it = -1;  // sweeps done on codeword w; -1: not loaded (the same in every lane of the group)
+    while (w < a.count) {
+        if (it < 0) {
+            for (int p = q; p < c.n; p += G) {
+                T v;
+                if (a.input_full) {
+                    v = a.llr[w * c.n + p];
+                } else {
+                    v = T(0);
+                    for (uint32_t s = c.src_start[p]; s < c.src_start[p + 1]; ++s) v += a.llr[w * c.e + c.src[s]];
+                    if (p >= c.k_info && p < c.k) v = T(30);
+                }
+                post[size_t(p) * L] = v;
+            }
+            sync();
+            if (a.apriori) {
+                for (int i = q; i < c.k_info; i += G) {
+                    const int p = c.info ? int(c.info[i]) : i;
+                    post[size_t(p) * L] += a.apriori[w * c.k_info + i];
+                }
+                sync();
+            }
+            it = 0;
+            if (a.max_iterations == 0 || !fails()) {
+                finish(w, 0);
+                sync();
+                w += slots;
+                it = -1;
+                continue;
+            }
+        }
+        for (int i = 0; i < c.rows_b; ++i) {
+            const uint32_t e0 = row_start[i];
+            const int deg = int(row_start[i + 1] - e0);
+            T in = T(0);
+            size_t at = 0;
+            if (q < deg) {
+                at = size_t(col[e0 + q]) * L;
+                const T old = it == 0 ? T(0) : c2v[size_t(e0 + q) * L];
+                in = post[at] - old;
+            }
+            const T out = group_check<G, T>(a.rule, T(a.alpha), in, q, deg);
+            if (q < deg) {
+                post[at] = in + out;
+                c2v[size_t(e0 + q) * L] = out;
+            }
+            sync();
+        }
+        ++it;
+        if (it >= a.max_iterations || !fails()) {
+            finish(w, it);
+            sync();
+            w += slots;
+            it = -1;
+        }
+    }
+}
+
 // Scratch of the lane-per-codeword kernel at most (its lanes are sized to fit).
 constexpr size_t kRowsScratchBytes = size_t(8) << 30;
 
@@ -417,9 +612,36 @@ hipError_t launch_rows(const GenericArgs<T>& a0, Workspace& ws, hipStream_t stre
     return hipGetLastError();
 }
 
+template <typename T, int G>
+hipError_t launch_groups(const GenericArgs<T>& a0, Workspace& ws, hipStream_t stream) {
+    GenericArgs<T> a = a0;
+    const DevCode& c = a.code;
+    const size_t per_slot = (size_t(c.n) + size_t(c.edges_b)) * sizeof(T);
+    int64_t slots = int64_t(kRowsScratchBytes / per_slot);
+    if (slots > a.count) slots = a.count;
+    if (slots < 1) return hipSuccess;
+    // fewer groups (each then decodes more codewords) when the device cannot hold them all
+    T* scratch = static_cast<T*>(ws.get(size_t(slots) * per_slot));
+    while (!scratch && slots > 64) {
+        slots /= 2;
+        scratch = static_cast<T*>(ws.get(size_t(slots) * per_slot));
+    }
+    if (!scratch) return hipErrorOutOfMemory;
+    a.scratch = scratch;
+    a.scratch_stride = 0;
+    const unsigned blocks = unsigned((slots * G + 63) / 64);
+    hipLaunchKernelGGL((decode_groups_kernel<T, G>), dim3(blocks), dim3(64), 0, stream, a, slots);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch(const GenericArgs<T>& a0, int device, hipStream_t stream, Workspace& ws) {
-    if (a0.layered && a0.code.z == 1) return launch_rows(a0, ws, stream);
+    if (a0.layered && a0.code.z == 1) {
+        // codes given by their rows: a group of lanes per codeword when a check fits in one
+        if (a0.code.max_deg <= 8) return launch_groups<T, 8>(a0, ws, stream);
+        if (a0.code.max_deg <= 16) return launch_groups<T, 16>(a0, ws, stream);
+        return launch_rows(a0, ws, stream);
+    }
     GenericArgs<T> a = a0;
     const DevCode& c = a.code;
     int threads = ((c.z + 63) / 64) * 64;

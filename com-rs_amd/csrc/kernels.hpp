@@ -23,6 +23,7 @@ struct DevCode {
     const uint32_t* info;       // k_info information positions, or null: 0 .. k_info - 1
     const uint32_t* row_start32;  // row_start and col as 32-bit words
     const uint32_t* col32;
+    int32_t max_deg;              // the widest check
 };
 
 template <typename T>
