@@ -507,12 +507,20 @@ __global__ void __launch_bounds__(64) decode_groups_kernel(GenericArgs<T> a, int
     // a lane reads what another lane of its wavefront wrote: in order within a wavefront
     auto sync = []() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); };
     // ldpc.rs:555-557, the same answer in every lane of the group
+    // (each check's column indices are read one check ahead, so that its posterior reads
+    // wait for one memory round trip, not two)
+    auto column = [&](int i) -> uint32_t {
+        const uint32_t e0 = row_start[i];
+        return q < int(row_start[i + 1] - e0) ? col[e0 + q] : 0u;
+    };
     auto fails = [&]() -> bool {
+        uint32_t v_next = column(0);
         for (int i = 0; i < c.rows_b; ++i) {
-            const uint32_t e0 = row_start[i];
-            const int deg = int(row_start[i + 1] - e0);
+            const int deg = int(row_start[i + 1] - row_start[i]);
+            const uint32_t v = v_next;
+            if (i + 1 < c.rows_b) v_next = column(i + 1);
             uint32_t bit = 0;
-            if (q < deg) bit = post[size_t(col[e0 + q]) * L] < T(0);
+            if (q < deg) bit = post[size_t(v) * L] < T(0);
             if (group_all<G>(bit, [](uint32_t x, uint32_t y) { return x ^ y; })) return true;
         }
         return false;
@@ -557,16 +565,24 @@ __global__ void __launch_bounds__(64) decode_groups_kernel(GenericArgs<T> a, int
                 continue;
             }
         }
+        // the column indices and the old messages of a check are read one check ahead (no
+        // check writes another's messages)
+        auto message = [&](int i) -> T {
+            const uint32_t e0 = row_start[i];
+            return it == 0 || q >= int(row_start[i + 1] - e0) ? T(0) : c2v[size_t(e0 + q) * L];
+        };
+        uint32_t v_next = column(0);
+        T old_next = message(0);
         for (int i = 0; i < c.rows_b; ++i) {
             const uint32_t e0 = row_start[i];
             const int deg = int(row_start[i + 1] - e0);
-            T in = T(0);
-            size_t at = 0;
-            if (q < deg) {
-                at = size_t(col[e0 + q]) * L;
-                const T old = it == 0 ? T(0) : c2v[size_t(e0 + q) * L];
-                in = post[at] - old;
+            const size_t at = size_t(v_next) * L;
+            const T old = old_next;
+            if (i + 1 < c.rows_b) {
+                v_next = column(i + 1);
+                old_next = message(i + 1);
             }
+            const T in = q < deg ? post[at] - old : T(0);
             const T out = group_check<G, T>(a.rule, T(a.alpha), in, q, deg);
             if (q < deg) {
                 post[at] = in + out;
