@@ -392,9 +392,10 @@ __global__ void __launch_bounds__(64) decode_rows_kernel(GenericArgs<T> a, int64
 // ---- codes given by their rows, a group of G lanes per codeword ------------------------
 //
 // The same schedule as decode_rows_kernel, with the edges of a check spread over a group of
-// G = 8 or 16 lanes (edge q on lane q of the group): G times as many codewords in flight for
-// a batch, and a check's work done side by side. The groups sit inside the 16-lane rows of
-// the wavefront, so their exchanges are DPP moves: the check's minimum, first minimum and
+// G = 8, 16 or 32 lanes (edge q on lane q of the group): G times as many codewords in flight
+// for a batch, and a check's work done side by side. Groups of 8 and 16 sit inside the
+// 16-lane rows of the wavefront, so their exchanges are DPP moves (groups of 32 cross one
+// row boundary through ds_swizzle): the check's minimum, first minimum and
 // sign parity by butterflies (order-free: a minimum is a selection), the sum-product's
 // prefix and suffix by sequential shifts in the reference's order, so that every result is
 // the one check_row computes. State laid out [index][group].
@@ -421,32 +422,60 @@ __device__ __forceinline__ uint32_t dpp(uint32_t x) {
 constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kHalfMirror = 0x141, kRowMirror = 0x140;
 constexpr int kRowShr1 = 0x111, kRowShl1 = 0x101;
 
+// Groups of 32 lanes span two DPP rows: the crossing goes through ds_swizzle within each
+// 32 lanes (bit mode: lane' = ((lane & and) | or) ^ xor): lane ^ 16, lane 15, lane 16.
+constexpr int kSwapRows = (0x10 << 10) | 0x1F, kFromLane15 = 15 << 5, kFromLane16 = 16 << 5;
+
+template <int PATTERN>
+__device__ __forceinline__ uint32_t swz32(uint32_t x) {
+    return uint32_t(__builtin_amdgcn_ds_swizzle(int(x), PATTERN));
+}
+template <int PATTERN>
+__device__ __forceinline__ float swz(float x) {
+    return __uint_as_float(swz32<PATTERN>(__float_as_uint(x)));
+}
+template <int PATTERN>
+__device__ __forceinline__ double swz(double x) {
+    const uint64_t b = __double_as_longlong(x);
+    const uint64_t lo = swz32<PATTERN>(uint32_t(b)), hi = swz32<PATTERN>(uint32_t(b >> 32));
+    return __longlong_as_double(int64_t(lo | (hi << 32)));
+}
+template <int PATTERN>
+__device__ __forceinline__ uint32_t swz(uint32_t x) {
+    return swz32<PATTERN>(x);
+}
+
 // op over the G lanes of the group, the result in every lane of it
 template <int G, typename V, typename F>
 __device__ __forceinline__ V group_all(V x, F op) {
     x = op(x, dpp<kQuadXor1>(x));
     x = op(x, dpp<kQuadXor2>(x));
     x = op(x, dpp<kHalfMirror>(x));
-    if constexpr (G == 16) x = op(x, dpp<kRowMirror>(x));
+    if constexpr (G >= 16) x = op(x, dpp<kRowMirror>(x));
+    if constexpr (G == 32) x = op(x, swz<kSwapRows>(x));
     return x;
 }
 
 // exclusive scan in lane order: lane q gets ((id op v_0) op v_1) ... op v_{q-1}
-template <typename V, typename F>
+template <int G, typename V, typename F>
 __device__ __forceinline__ V group_prefix(V v, V id, int q, int deg, F op) {
     V pre = id;
     for (int k = 1; k < deg; ++k) {
-        const V cand = dpp<kRowShr1>(op(pre, v));
+        V cand;
+        if (G == 32 && k == 16) cand = swz<kFromLane15>(op(pre, v));
+        else cand = dpp<kRowShr1>(op(pre, v));
         pre = q == k ? cand : pre;
     }
     return pre;
 }
 // and from the end: lane q gets ((id op v_{deg-1}) op v_{deg-2}) ... op v_{q+1}
-template <typename V, typename F>
+template <int G, typename V, typename F>
 __device__ __forceinline__ V group_suffix(V v, V id, int q, int deg, F op) {
     V suf = id;
     for (int k = deg - 2; k >= 0; --k) {
-        const V cand = dpp<kRowShl1>(op(suf, v));
+        V cand;
+        if (G == 32 && k == 15) cand = swz<kFromLane16>(op(suf, v));
+        else cand = dpp<kRowShl1>(op(suf, v));
         suf = q == k ? cand : suf;
     }
     return suf;
@@ -459,16 +488,16 @@ __device__ __forceinline__ T group_check(int rule, T alpha, T in, int q, int deg
     if (rule == 0) {
         if constexpr (sizeof(T) == 4) {
             const float s = act ? spa_phi(fabsf(in)) : 0.0f;
-            const float pre = group_prefix(s, 0.0f, q, deg, [](float x, float y) { return x + y; });
-            const float suf = group_suffix(s, 0.0f, q, deg, [](float x, float y) { return x + y; });
+            const float pre = group_prefix<G>(s, 0.0f, q, deg, [](float x, float y) { return x + y; });
+            const float suf = group_suffix<G>(s, 0.0f, q, deg, [](float x, float y) { return x + y; });
             const uint32_t parity = group_all<G>(act ? __float_as_uint(in) : 0u,
                                                  [](uint32_t x, uint32_t y) { return x ^ y; });
             const float m = fminf(spa_phi(pre + suf), kSpaCap);
             return __uint_as_float(__float_as_uint(m) | ((parity ^ __float_as_uint(in)) & 0x80000000u));
         } else {
             const T t = act ? tanh_t(in / T(2)) : T(1);
-            const T pre = group_prefix(t, T(1), q, deg, [](T x, T y) { return x * y; });
-            const T suf = group_suffix(t, T(1), q, deg, [](T x, T y) { return x * y; });
+            const T pre = group_prefix<G>(t, T(1), q, deg, [](T x, T y) { return x * y; });
+            const T suf = group_suffix<G>(t, T(1), q, deg, [](T x, T y) { return x * y; });
             const T bound = T(0.999999999999);
             T p = pre * suf;
             p = p < -bound ? -bound : (p > bound ? bound : p);
@@ -656,6 +685,7 @@ hipError_t launch(const GenericArgs<T>& a0, int device, hipStream_t stream, Work
         // codes given by their rows: a group of lanes per codeword when a check fits in one
         if (a0.code.max_deg <= 8) return launch_groups<T, 8>(a0, ws, stream);
         if (a0.code.max_deg <= 16) return launch_groups<T, 16>(a0, ws, stream);
+        if (a0.code.max_deg <= 32) return launch_groups<T, 32>(a0, ws, stream);
         return launch_rows(a0, ws, stream);
     }
     GenericArgs<T> a = a0;

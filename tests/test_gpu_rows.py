@@ -228,3 +228,20 @@ def test_layered_groups_sum_product_f32():
     rel = np.abs(post - p32) / np.maximum(np.abs(p32), 1.0)
     assert np.median(rel) < 1e-6
     np.testing.assert_allclose(post, p32, rtol=1e-3, atol=5e-2)
+
+
+def test_layered_groups_of_32_bit_exact():
+    """Checks of 17 to 19 edges (the PEG (648, 540) code) take a group of 32 lanes, whose
+    exchanges cross a DPP row through ds_swizzle: layered normalized min-sum f64 and FP32
+    posteriors and iteration counts bit-identical to the oracle's."""
+    print("\n[rows] groups of 32, layered NMS", flush=True)
+    n, rows, code, ref = setup("peg_n648_k540.alist", "normalized_min_sum", 0.75, True, 20)
+    llr = zero_codeword_llrs("groups32", n, 3.5, 540 / 648, 101, cond=5)
+    post_ref, it_ref = ref.decode_many(llr, rule=O.NMS, alpha=0.75, layered=True, max_iterations=20, threads=16)
+    _, its, post = code.decode_batch(torch.from_numpy(llr).cuda(), want_post=True)
+    assert np.array_equal(its.cpu().numpy(), it_ref) and np.array_equal(post.cpu().numpy(), post_ref)
+    p32, i32 = ref.decode_many_f32(llr.astype(np.float32), rule=O.NMS, alpha=0.75, layered=True,
+                                   max_iterations=20, threads=16)
+    _, its32, post32 = code.decode_batch(torch.from_numpy(llr.astype(np.float32)).cuda(), want_post=True)
+    assert np.array_equal(its32.cpu().numpy(), i32) and np.array_equal(post32.cpu().numpy(), p32)
+    assert len(set(it_ref.tolist())) >= 3
