@@ -93,7 +93,17 @@ __global__ void __launch_bounds__(384) learn_forward_kernel(LearnArgs a) {
     }
     __syncthreads();
     float loss = 0.0f;
+    // the checkpoint of iteration t-1 of row i (read one row ahead: nothing of this
+    // iteration writes it, so its reads need not wait for the barrier)
+    auto load_prev = [&](int t, int i, float (&dst)[kMaxDeg]) {
+        const int e0 = c.row_start[i], deg = c.row_start[i + 1] - e0;
+#pragma unroll
+        for (int q = 0; q < kMaxDeg; ++q)
+            if (q < deg) dst[q] = ckpt[size_t(t - 1) * edges + size_t(e0 + q) * c.z + r];
+    };
     for (int t = 0; t < a.T; ++t) {
+        float prev_next[kMaxDeg] = {};
+        if (t > 0 && r < c.z) load_prev(t, 0, prev_next);
         for (int i = 0; i < c.rows_b; ++i) {
             if (r < c.z) {
                 const int e0 = c.row_start[i], deg = c.row_start[i + 1] - e0;
@@ -104,8 +114,8 @@ __global__ void __launch_bounds__(384) learn_forward_kernel(LearnArgs a) {
                     // the row's messages of iteration t-1, recomputed from its checkpoint
                     float prev[kMaxDeg];
 #pragma unroll
-                    for (int q = 0; q < kMaxDeg; ++q)
-                        if (q < deg) prev[q] = ckpt[size_t(t - 1) * edges + size_t(e0 + q) * c.z + r];
+                    for (int q = 0; q < kMaxDeg; ++q) prev[q] = prev_next[q];
+                    if (i + 1 < c.rows_b) load_prev(t, i + 1, prev_next);
                     const RowScan s = scan_row(prev, deg);
                     const float pa = a.params[((t - 1) * c.rows_b + i) * 2], pb = a.params[((t - 1) * c.rows_b + i) * 2 + 1];
                     const float g1 = pa * fmaxf(s.m1 - pb, 0.0f), g2 = pa * fmaxf(s.m2 - pb, 0.0f);
@@ -175,18 +185,34 @@ __global__ void __launch_bounds__(384) learn_backward_kernel(LearnArgs a) {
         const float* gpost = a.gpost + (size_t(w) * a.T + t) * c.n;
         for (int p = threadIdx.x; p < c.n; p += blockDim.x) pbar[p] += gpost[p];
         __syncthreads();
+        // a row's checkpoint and its message adjoints of iteration t+1 are read one row
+        // ahead: nothing of this iteration before the row itself writes them
+        float in_next[kMaxDeg] = {}, cb_next[kMaxDeg] = {};
+        auto load_row = [&](int i) {
+            const int e0 = c.row_start[i], deg = c.row_start[i + 1] - e0;
+#pragma unroll
+            for (int q = 0; q < kMaxDeg; ++q)
+                if (q < deg) {
+                    in_next[q] = ckpt[size_t(t) * edges + size_t(e0 + q) * c.z + r];
+                    cb_next[q] = t < a.T - 1 ? cbar[size_t(e0 + q) * c.z + r] : 0.0f;
+                }
+        };
+        if (r < c.z) load_row(c.rows_b - 1);
         for (int i = c.rows_b - 1; i >= 0; --i) {
             float ga = 0.0f, gb = 0.0f;
             if (r < c.z) {
                 const int e0 = c.row_start[i], deg = c.row_start[i + 1] - e0;
-                float in[kMaxDeg];
+                float in[kMaxDeg], cb[kMaxDeg];
                 int addr[kMaxDeg];
 #pragma unroll
+                for (int q = 0; q < kMaxDeg; ++q) {
+                    in[q] = in_next[q];
+                    cb[q] = cb_next[q];
+                }
+                if (i > 0) load_row(i - 1);
+#pragma unroll
                 for (int q = 0; q < kMaxDeg; ++q)
-                    if (q < deg) {
-                        in[q] = ckpt[size_t(t) * edges + size_t(e0 + q) * c.z + r];
-                        addr[q] = int(c.col[e0 + q]) * c.z + wrap_add(r, c.shift[e0 + q], c.z);
-                    }
+                    if (q < deg) addr[q] = int(c.col[e0 + q]) * c.z + wrap_add(r, c.shift[e0 + q], c.z);
                 const RowScan s = scan_row(in, deg);
                 const float pa = a.params[(t * c.rows_b + i) * 2], pb = a.params[(t * c.rows_b + i) * 2 + 1];
                 float ibar[kMaxDeg];
@@ -194,7 +220,7 @@ __global__ void __launch_bounds__(384) learn_backward_kernel(LearnArgs a) {
 #pragma unroll
                 for (int q = 0; q < kMaxDeg; ++q)
                     if (q < deg) {
-                        const float ob = pbar[addr[q]] + (t < a.T - 1 ? cbar[size_t(e0 + q) * c.z + r] : 0.0f);
+                        const float ob = pbar[addr[q]] + cb[q];
                         const float m = q == s.i1 ? s.m2 : s.m1;
                         const float sign = ((s.parity ^ (s.neg >> q)) & 1u) ? -1.0f : 1.0f;
                         const float active = m > pb ? 1.0f : 0.0f;
