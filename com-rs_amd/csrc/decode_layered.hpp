@@ -255,11 +255,13 @@ __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post
     // one base address per column (68 of them) in registers through the whole iteration.
     // With Z known at compile time (ZC) the column bases are immediate offsets.
     uint32_t base = 0;
+    constexpr uint32_t es0 = sizeof(typename S::T);
+    uint32_t rb = uint32_t(r) * es0;  // the row's byte offset in a column (ZC)
     if constexpr (ZC != 0) {
-        // the shifts are constants too, so the row index is what keeps the addresses (all
+        // the shifts are constants too, so the row offset is what keeps the addresses (all
         // 316 of them loop-invariant) from being computed once and held in registers
         z = ZC;
-        asm volatile("" : "+v"(r));
+        asm volatile("" : "+v"(rb));
     } else {
         asm volatile("" : "+s"(z));
     }
@@ -289,7 +291,7 @@ __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post
         const int e = e0 + q;
         if constexpr (ZC != 0) {
             const uint32_t s_q = shift_of<BG, ZC>(e) * es;
-            const uint32_t t = uint32_t(r) * es + s_q;
+            const uint32_t t = rb + s_q;
             const uint32_t rr = s_q == 0 ? t : min(t, t - uint32_t(ZC) * es);
             return int(rr) + int(Graph<BG>::t[e].j) * ZC * int(es);
         } else {
