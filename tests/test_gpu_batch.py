@@ -214,16 +214,17 @@ def test_uneven_batch_bit_exact_per_codeword(half):
     assert torch.equal(b2, result[0]) and torch.equal(i2, result[1]) and torch.equal(o2, result[2])
 
 
-def test_uneven_20000_codewords_f16_per_codeword():
-    """The uneven workload at full size: 20 000 codewords, one persistent launch, FP16
-    storage, every codeword compared with the oracle's binary16 restatement."""
-    print("\n[uneven batch] 20000 codewords f16", flush=True)
+@pytest.mark.parametrize("half", [True, False], ids=["f16", "f32"])
+def test_uneven_20000_codewords_per_codeword(half):
+    """The uneven workload at full size: 20 000 codewords, one persistent launch, FP16 or
+    FP32 storage, every codeword compared with the oracle's binary16 or FP32 restatement."""
+    print(f"\n[uneven batch] 20000 codewords half={half}", flush=True)
     codes, desc, flat, offsets, fulls = uneven_workload(20000, seed=7)
-    plan = F.BatchPlan(codes, offsets, desc[2], half=True)
+    plan = F.BatchPlan(codes, offsets, desc[2], half=half)
     result = plan.run(flat)
     torch.cuda.synchronize()
     print(f"  decoded; grid {plan.dims.grid} x {plan.dims.threads} lanes, {plan.dims.tasks} tasks", flush=True)
-    check_against_oracle(codes, desc, fulls, result, True)
+    check_against_oracle(codes, desc, fulls, result, half)
 
 
 def test_width_autotuner_and_graph_replays():
