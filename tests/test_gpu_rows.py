@@ -245,3 +245,22 @@ def test_layered_groups_of_32_bit_exact():
     _, its32, post32 = code.decode_batch(torch.from_numpy(llr.astype(np.float32)).cuda(), want_post=True)
     assert np.array_equal(its32.cpu().numpy(), i32) and np.array_equal(post32.cpu().numpy(), p32)
     assert len(set(it_ref.tolist())) >= 3
+
+
+def test_groups_edge_cases():
+    """The group kernel with an empty batch, a limit of 0 iterations (the channel LLRs come
+    back, 0 iterations), a codeword given as input (0 iterations) and a batch of one."""
+    print("\n[rows] group kernel edge cases", flush=True)
+    n, rows, code, ref = setup("peg_n576_k288.alist", "normalized_min_sum", 0.75, True, 0)
+    bits, its, post = code.decode_batch(torch.zeros((0, n), dtype=torch.float64, device="cuda"), want_post=True)
+    assert bits.shape == (0, code.k) and its.shape == (0,) and post.shape == (0, n)
+    llr = zero_codeword_llrs("groups_edge", n, 0.5, 0.5, 5, cond=6)
+    _, its, post = code.decode_batch(torch.from_numpy(llr).cuda(), want_post=True)
+    assert (its.cpu().numpy() == 0).all() and np.array_equal(post.cpu().numpy(), llr)
+    code.set_decoder("normalized_min_sum", 0.75, True, 20)
+    clean = np.full((3, n), 4.0)
+    _, its, post = code.decode_batch(torch.from_numpy(clean).cuda(), want_post=True)
+    assert (its.cpu().numpy() == 0).all() and np.array_equal(post.cpu().numpy(), clean)
+    post_ref, it_ref = ref.decode_llr(llr[1], rule=O.NMS, alpha=0.75, layered=True, max_iterations=20)
+    _, its, post = code.decode_batch(torch.from_numpy(llr[1:2].copy()).cuda(), want_post=True)
+    assert its.item() == it_ref and np.array_equal(post.cpu().numpy()[0], post_ref)
