@@ -66,6 +66,7 @@ DevCode dev_code(const NrCode& c, int device) {
     d.row_start32 = t.row_start32;
     d.col32 = t.col32;
     d.max_deg = int32_t(c.max_row_degree());
+    d.distinct = c.rows_distinct() ? 1 : 0;
     return d;
 }
 

@@ -309,6 +309,7 @@ __global__ void __launch_bounds__(64) decode_rows_kernel(GenericArgs<T> a, int64
     if (gid >= lanes) return;
     const size_t L = size_t(lanes);
     // the row tables as 32-bit words: every lane reads the same entry, so these are scalar loads
+    // (a column entry holds the variable in its low 16 bits)
     const uint32_t* __restrict__ row_start = c.row_start32;
     const uint32_t* __restrict__ col = c.col32;
     T* post = a.scratch + gid;                       // post[p * L], p < n
@@ -317,7 +318,7 @@ __global__ void __launch_bounds__(64) decode_rows_kernel(GenericArgs<T> a, int64
     auto fails = [&]() -> bool {
         for (int i = 0; i < c.rows_b; ++i) {
             uint32_t par = 0;
-            for (uint32_t e = row_start[i]; e < row_start[i + 1]; ++e) par ^= post[size_t(col[e]) * L] < T(0);
+            for (uint32_t e = row_start[i]; e < row_start[i + 1]; ++e) par ^= post[size_t(col[e] & 0xffffu) * L] < T(0);
             if (par) return true;
         }
         return false;
@@ -368,7 +369,7 @@ __global__ void __launch_bounds__(64) decode_rows_kernel(GenericArgs<T> a, int64
 #pragma unroll
             for (int q = 0; q < kMaxDeg; ++q)
                 if (q < deg) {
-                    at[q] = size_t(col[e0 + q]) * L;
+                    at[q] = size_t(col[e0 + q] & 0xffffu) * L;
                     const T old = it == 0 ? T(0) : c2v[size_t(e0 + q) * L];
                     in[q] = post[at[q]] - old;
                 }
@@ -481,6 +482,17 @@ __device__ __forceinline__ V group_suffix(V v, V id, int q, int deg, F op) {
     return suf;
 }
 
+// x of wavefront lane `lane` (ds_bpermute: any lane, chosen per lane)
+__device__ __forceinline__ float lane_value(float x, int lane) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(lane << 2, __float_as_int(x)));
+}
+__device__ __forceinline__ double lane_value(double x, int lane) {
+    const uint64_t b = __double_as_longlong(x);
+    const uint64_t lo = uint32_t(__builtin_amdgcn_ds_bpermute(lane << 2, int(uint32_t(b))));
+    const uint64_t hi = uint32_t(__builtin_amdgcn_ds_bpermute(lane << 2, int(uint32_t(b >> 32))));
+    return __longlong_as_double(int64_t(lo | (hi << 32)));
+}
+
 // check_row for the edge of lane q (the lanes q >= deg take part with neutral values)
 template <int G, typename T>
 __device__ __forceinline__ T group_check(int rule, T alpha, T in, int q, int deg) {
@@ -522,6 +534,21 @@ __device__ __forceinline__ T group_check(int rule, T alpha, T in, int q, int deg
     return (neg != 0) != (in < T(0)) ? -g : g;
 }
 
+constexpr uint32_t kNoColumn = 0xffffffffu;  // the column of a lane past the check's edges
+
+// One check in flight: its edges (read three checks ahead), its posteriors and old messages
+// (read two checks ahead) and what the checks in between wrote to its columns.
+template <typename T>
+struct Pending {
+    uint32_t e0;
+    int deg;
+    uint32_t v;   // the lane's column, kNoColumn past the edges
+    uint32_t from;  // where the two checks before list the column (bits 0..5, 6..11), 0 past the edges
+    T p, old;     // the posterior as read, the message of the previous sweep
+    uint32_t fresh;  // a check after the read wrote the column (1): its value is fv
+    T fv;
+};
+
 template <typename T, int G>
 __global__ void __launch_bounds__(64) decode_groups_kernel(GenericArgs<T> a, int64_t slots) {
     const DevCode& c = a.code;
@@ -529,28 +556,49 @@ __global__ void __launch_bounds__(64) decode_groups_kernel(GenericArgs<T> a, int
     const int q = int(threadIdx.x) & (G - 1);
     if (slot >= slots) return;  // whole groups: G divides the workgroup
     const size_t L = size_t(slots);
+    // element offsets in 32 bits: the scratch holds fewer than 2^32 values (at most 8 GiB)
+    const uint32_t Lu = uint32_t(slots);
+    auto at = [&](uint32_t index) -> size_t { return size_t(index * Lu); };
+    const int rows = c.rows_b;
     const uint32_t* __restrict__ row_start = c.row_start32;
     const uint32_t* __restrict__ col = c.col32;
     T* post = a.scratch + slot;                      // post[p * L], p < n
     T* c2v = a.scratch + size_t(c.n) * L + slot;     // c2v[e * L], e < edges
+    T* sink = c2v + at(uint32_t(c.edges_b));           // where the lanes past a check's edges store
     // a lane reads what another lane of its wavefront wrote: in order within a wavefront
     auto sync = []() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); };
-    // ldpc.rs:555-557, the same answer in every lane of the group
-    // (each check's column indices are read one check ahead, so that its posterior reads
-    // wait for one memory round trip, not two)
-    auto column = [&](int i) -> uint32_t {
-        const uint32_t e0 = row_start[i];
-        return q < int(row_start[i + 1] - e0) ? col[e0 + q] : 0u;
+    // Every read below is unconditional (a lane past its check's edges reads entry 0 and
+    // drops the value), so that no branch separates a read from the next one and the reads
+    // of several checks go out together. The check index is made wavefront-uniform and the
+    // row starts read through the constant address space: scalar reads.
+    using ConstU32 = const __attribute__((address_space(4))) uint32_t;
+    ConstU32* row_start_s = (ConstU32*)row_start;  // the table is not written while the kernel runs
+    auto row = [&](int i, uint32_t& e0, int& deg) {
+        const int iu = __builtin_amdgcn_readfirstlane(i);
+        const bool in_range = uint32_t(iu) < uint32_t(rows);  // checks before 0 and past the last: none
+        const int ic = in_range ? iu : 0;
+        e0 = row_start_s[ic];
+        const uint32_t e1 = row_start_s[ic + 1];
+        deg = in_range ? int(e1 - e0) : 0;
     };
+    // ldpc.rs:555-557, the same answer in every lane of the group: 32 checks at a time, their
+    // columns read together, then their posteriors, the parities packed one bit per check
+    // (the XOR over the group reduces all 32 at once)
     auto fails = [&]() -> bool {
-        uint32_t v_next = column(0);
-        for (int i = 0; i < c.rows_b; ++i) {
-            const int deg = int(row_start[i + 1] - row_start[i]);
-            const uint32_t v = v_next;
-            if (i + 1 < c.rows_b) v_next = column(i + 1);
-            uint32_t bit = 0;
-            if (q < deg) bit = post[size_t(v) * L] < T(0);
-            if (group_all<G>(bit, [](uint32_t x, uint32_t y) { return x ^ y; })) return true;
+        for (int i0 = 0; i0 < rows; i0 += 32) {
+            uint32_t e0[32], v[32];
+            int deg[32];
+#pragma unroll
+            for (int j = 0; j < 32; ++j) row(i0 + j, e0[j], deg[j]);
+#pragma unroll
+            for (int j = 0; j < 32; ++j) v[j] = col[q < deg[j] ? e0[j] + q : 0u] & 0xffffu;
+            uint32_t bits = 0;
+#pragma unroll
+            for (int j = 0; j < 32; ++j) {
+                const T pv = post[at(q < deg[j] ? v[j] : 0u)];
+                bits |= q < deg[j] && pv < T(0) ? 1u << j : 0u;
+            }
+            if (group_all<G>(bits, [](uint32_t x, uint32_t y) { return x ^ y; })) return true;
         }
         return false;
     };
@@ -564,6 +612,55 @@ __global__ void __launch_bounds__(64) decode_groups_kernel(GenericArgs<T> a, int
     };
     int64_t w = slot;
     int it = -1;  // sweeps done on codeword w; -1: not loaded (the same in every lane of the group)
+    // A check reads the posteriors that the check before it has just written (always so in
+    // a staircase), so a sweep that read them only then would wait a memory round trip per
+    // check. They are read two checks ahead instead, before the two checks in between store
+    // theirs; each of those then hands what it wrote to the lanes of the checks in flight
+    // whose column it is, the newer over the older. The column table says, per edge, which
+    // lane of each of the two checks before holds the same column (columns are distinct
+    // within a check), so a hand-over is one lane permutation.
+    const int group_lane0 = int(threadIdx.x) & 63 & ~(G - 1);
+    auto head = [&](Pending<T>& s, int i) {
+        row(i, s.e0, s.deg);
+        const uint32_t cv = col[q < s.deg ? s.e0 + q : 0u];
+        s.v = q < s.deg ? cv & 0xffffu : kNoColumn;
+        s.from = q < s.deg ? cv >> 16 : 0u;
+    };
+    // the posterior and the message of the previous sweep (dropped on the first sweep where used)
+    auto fetch = [&](Pending<T>& s) {
+        s.fresh = 0;
+        s.fv = T(0);
+        s.p = post[at(q < s.deg ? s.v : 0u)];
+        s.old = c2v[at(q < s.deg ? s.e0 + q : 0u)];
+    };
+    // check i: cur; i + 1 and i + 2 in flight (n1, n2: the reads of i + 2 go out here);
+    // n3 takes the edges of i + 3
+    auto step = [&](int i, Pending<T>& cur, Pending<T>& n1, Pending<T>& n2, Pending<T>& n3) {
+        head(n3, i + 3);
+        fetch(n2);
+        const int deg = cur.deg;
+        const T p = cur.fresh ? cur.fv : cur.p;
+        const T in = q < deg ? p - (it > 0 ? cur.old : T(0)) : T(0);
+        const T out = group_check<G, T>(a.rule, T(a.alpha), in, q, deg);
+        const T np = in + out;
+        // (stored by every lane, so that the wait for a later read counts the same stores on
+        // every path)
+        *(q < deg ? post + at(cur.v) : sink) = np;
+        *(q < deg ? c2v + at(cur.e0 + q) : sink) = out;
+        // to check i + 1 from the lane its entry names for the check before, to check i + 2
+        // from the one named for the check two before
+        const T x1 = lane_value(np, group_lane0 + int(n1.from & 31u));
+        const T x2 = lane_value(np, group_lane0 + int((n2.from >> 6) & 31u));
+        const bool h1 = (n1.from & 32u) != 0, h2 = (n2.from & 2048u) != 0;
+        n1.fresh = h1 ? 1u : n1.fresh;
+        n1.fv = h1 ? x1 : n1.fv;
+        n2.fresh = h2 ? 1u : n2.fresh;
+        n2.fv = h2 ? x2 : n2.fv;
+        // kept apart from the reads in flight: folded into them, the selects would wait for them
+        asm volatile("" : "+v"(n1.fresh), "+v"(n1.fv), "+v"(n2.fresh), "+v"(n2.fv));
+        sync();
+    };
+    Pending<T> s0, s1, s2, s3;
     while (w < a.count) {
         if (it < 0) {
             for (int p = q; p < c.n; p += G) {
@@ -594,30 +691,27 @@ __global__ void __launch_bounds__(64) decode_groups_kernel(GenericArgs<T> a, int
                 continue;
             }
         }
-        // the column indices and the old messages of a check are read one check ahead (no
-        // check writes another's messages)
-        auto message = [&](int i) -> T {
-            const uint32_t e0 = row_start[i];
-            return it == 0 || q >= int(row_start[i + 1] - e0) ? T(0) : c2v[size_t(e0 + q) * L];
-        };
-        uint32_t v_next = column(0);
-        T old_next = message(0);
-        for (int i = 0; i < c.rows_b; ++i) {
-            const uint32_t e0 = row_start[i];
-            const int deg = int(row_start[i + 1] - e0);
-            const size_t at = size_t(v_next) * L;
-            const T old = old_next;
-            if (i + 1 < c.rows_b) {
-                v_next = column(i + 1);
-                old_next = message(i + 1);
-            }
-            const T in = q < deg ? post[at] - old : T(0);
-            const T out = group_check<G, T>(a.rule, T(a.alpha), in, q, deg);
-            if (q < deg) {
-                post[at] = in + out;
-                c2v[size_t(e0 + q) * L] = out;
-            }
-            sync();
+        // one sweep, ldpc.rs:522-531 for checks 0, 1, ..., m - 1 (the lanes past a check's
+        // edges hold the column kNoColumn, which is no variable: what they take is unused)
+        // The sweep starts 4 steps early, on checks -4 .. -1 that have no edges, so that its
+        // first reads go out inside the loop as every later read does (the compiler's count of
+        // the reads in flight is then the same on entry as around the loop), and ends on up to
+        // 3 such steps past the last check; taking them keeps every slot in its registers, with
+        // no copy that would wait for a read. Steps without edges read entry 0 and store to the
+        // sink; no entry names them, so they hand nothing on.
+        for (Pending<T>* e : {&s0, &s1, &s2, &s3}) {
+            e->e0 = 0;
+            e->deg = 0;
+            e->v = kNoColumn;
+            e->from = 0;
+            e->p = e->old = e->fv = T(0);
+            e->fresh = 0;
+        }
+        for (int i = -4; i < rows; i += 4) {
+            step(i, s0, s1, s2, s3);
+            step(i + 1, s1, s2, s3, s0);
+            step(i + 2, s2, s3, s0, s1);
+            step(i + 3, s3, s0, s1, s2);
         }
         ++it;
         if (it >= a.max_iterations || !fails()) {
@@ -661,7 +755,8 @@ template <typename T, int G>
 hipError_t launch_groups(const GenericArgs<T>& a0, Workspace& ws, hipStream_t stream) {
     GenericArgs<T> a = a0;
     const DevCode& c = a.code;
-    const size_t per_slot = (size_t(c.n) + size_t(c.edges_b)) * sizeof(T);
+    // posteriors, messages and one value the lanes past a check's edges write
+    const size_t per_slot = (size_t(c.n) + size_t(c.edges_b) + 1) * sizeof(T);
     int64_t slots = int64_t(kRowsScratchBytes / per_slot);
     if (slots > a.count) slots = a.count;
     if (slots < 1) return hipSuccess;
@@ -683,6 +778,9 @@ template <typename T>
 hipError_t launch(const GenericArgs<T>& a0, int device, hipStream_t stream, Workspace& ws) {
     if (a0.layered && a0.code.z == 1) {
         // codes given by their rows: a group of lanes per codeword when a check fits in one
+        // and lists no variable twice (else the edges of a check must be stored in order;
+        // the group kernel also reads entry 0 of the edge tables, so it needs one edge)
+        if (!a0.code.distinct || a0.code.edges_b == 0) return launch_rows(a0, ws, stream);
         if (a0.code.max_deg <= 8) return launch_groups<T, 8>(a0, ws, stream);
         if (a0.code.max_deg <= 16) return launch_groups<T, 16>(a0, ws, stream);
         if (a0.code.max_deg <= 32) return launch_groups<T, 32>(a0, ws, stream);

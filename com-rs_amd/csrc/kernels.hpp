@@ -24,6 +24,7 @@ struct DevCode {
     const uint32_t* row_start32;  // row_start and col as 32-bit words
     const uint32_t* col32;
     int32_t max_deg;              // the widest check
+    int32_t distinct;             // no check lists a variable twice
 };
 
 template <typename T>

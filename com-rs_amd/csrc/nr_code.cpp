@@ -277,6 +277,16 @@ uint32_t NrCode::max_row_degree() const {
     return d;
 }
 
+bool NrCode::rows_distinct() const {
+    std::vector<uint32_t> seen(cols_b, UINT32_MAX);
+    for (uint32_t i = 0; i < rows_b; ++i)
+        for (uint32_t e = row_start[i]; e < row_start[i + 1]; ++e) {
+            if (seen[col[e]] == i) return false;
+            seen[col[e]] = i;
+        }
+    return true;
+}
+
 bool NrCode::is_codeword(const uint8_t* cw) const {
     for (uint32_t i = 0; i < rows_b; ++i)
         for (uint32_t r = 0; r < z; ++r) {
@@ -372,8 +382,28 @@ const NrCode::DeviceTables& NrCode::device_tables(int device) const {
     std::memcpy(h_shift + shift.size(), info.data(), info.size() * 4);
     uint32_t* h_rs = h_shift + shift.size() + info.size();
     for (size_t i = 0; i < row_start.size(); ++i) h_rs[i] = row_start[i];
+    // the columns, each with where the two checks before list the same variable (bits 16..21:
+    // the edge's position in the check before, bit 21 set when it is there; bits 22..27 the
+    // same for the check two before); the last position of a variable listed twice
     uint32_t* h_col = h_rs + row_start.size();
-    for (size_t i = 0; i < col.size(); ++i) h_col[i] = col[i];
+    std::vector<int> pos1(cols_b, -1), pos2(cols_b, -1);  // positions in checks i - 1 and i - 2
+    auto mark = [&](std::vector<int>& pos, uint32_t r, bool set) {
+        for (uint32_t e = row_start[r]; e < row_start[r + 1]; ++e) pos[col[e]] = set ? int(e - row_start[r]) : -1;
+    };
+    for (uint32_t i = 0; i < rows_b; ++i) {
+        for (uint32_t e = row_start[i]; e < row_start[i + 1]; ++e) {
+            uint32_t w = col[e];
+            if (pos1[col[e]] >= 0) w |= (uint32_t(pos1[col[e]]) | 32u) << 16;
+            if (pos2[col[e]] >= 0) w |= (uint32_t(pos2[col[e]]) | 32u) << 22;
+            h_col[e] = w;
+        }
+        if (i >= 2) mark(pos2, i - 2, false);
+        if (i >= 1) {
+            mark(pos2, i - 1, true);
+            mark(pos1, i - 1, false);
+        }
+        mark(pos1, i, true);
+    }
     uint64_t* h64 = reinterpret_cast<uint64_t*>(host.data() + n16 * 2 + n32 * 4);
     std::memcpy(h64, core_inv.data(), core_inv.size() * 8);
     int prev = 0;

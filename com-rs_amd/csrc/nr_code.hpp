@@ -70,6 +70,8 @@ struct NrCode {
     static NrCode with_options(uint32_t k_info, uint32_t e, int bg, int rv, uint32_t qm);
 
     uint32_t max_row_degree() const;
+    // no check lists a variable twice (from_rows accepts such rows, as the reference does)
+    bool rows_distinct() const;
     // H c = 0 on the lifted matrix
     bool is_codeword(const uint8_t* cw) const;
     // LdpcCode::encode_block (ldpc.rs:753-791) on K systematic bits -> N bits

@@ -264,3 +264,25 @@ def test_groups_edge_cases():
     post_ref, it_ref = ref.decode_llr(llr[1], rule=O.NMS, alpha=0.75, layered=True, max_iterations=20)
     _, its, post = code.decode_batch(torch.from_numpy(llr[1:2].copy()).cuda(), want_post=True)
     assert its.item() == it_ref and np.array_equal(post.cpu().numpy()[0], post_ref)
+
+
+def test_checks_listing_a_variable_twice():
+    """A check may list a variable twice (the reference's from_rows takes such rows): the
+    device then stores a check's posteriors edge after edge, so that the later edge's value
+    stays, as in the oracle. The PEG (576, 288) code with the first variable of every
+    seventh check listed again at its end, layered NMS 0.75, 20 iterations: f64 and FP32
+    posteriors and iteration counts bit-identical to the oracle's."""
+    print("\n[rows] checks listing a variable twice", flush=True)
+    n, rows = read_alist(os.path.join(GOLDEN, "peg_n576_k288.alist"))
+    rows = [list(r) + ([r[0]] if i % 7 == 0 else []) for i, r in enumerate(rows)]
+    code = F.NrLdpc.from_rows(n, rows).set_decoder("normalized_min_sum", 0.75, True, 20)
+    ref = O.Code.from_rows(n, rows)
+    llr = zero_codeword_llrs("twice", n, 2.5, 0.5, 70, cond=7)
+    post_ref, it_ref = ref.decode_many(llr, rule=O.NMS, alpha=0.75, layered=True, max_iterations=20, threads=16)
+    _, its, post = code.decode_batch(torch.from_numpy(llr).cuda(), want_post=True)
+    assert np.array_equal(its.cpu().numpy(), it_ref) and np.array_equal(post.cpu().numpy(), post_ref)
+    p32, i32 = ref.decode_many_f32(llr.astype(np.float32), rule=O.NMS, alpha=0.75, layered=True,
+                                   max_iterations=20, threads=16)
+    _, its32, post32 = code.decode_batch(torch.from_numpy(llr.astype(np.float32)).cuda(), want_post=True)
+    assert np.array_equal(its32.cpu().numpy(), i32) and np.array_equal(post32.cpu().numpy(), p32)
+    assert len(set(it_ref.tolist())) >= 3
