@@ -403,7 +403,7 @@ __global__ void __launch_bounds__(64) decode_rows_kernel(GenericArgs<T> a, int64
 
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp32(uint32_t x) {
-    return uint32_t(__builtin_amdgcn_update_dpp(0, int(x), CTRL, 0xf, 0xf, false));
+    return uint32_t(__builtin_amdgcn_mov_dpp(int(x), CTRL, 0xf, 0xf, true));  // lanes without a source read 0
 }
 template <int CTRL>
 __device__ __forceinline__ float dpp(float x) {
