@@ -626,7 +626,7 @@ __global__ void __launch_bounds__(64) decode_groups_kernel(GenericArgs<T> a, int
         s.v = q < s.deg ? cv & 0xffffu : kNoColumn;
         s.from = q < s.deg ? cv >> 16 : 0u;
     };
-    // the posterior and the message of the previous sweep (dropped on the first sweep where used)
+    // the posterior and the message of the previous sweep (the latter unused on the first sweep)
     auto fetch = [&](Pending<T>& s) {
         s.fresh = 0;
         s.fv = T(0);

@@ -382,9 +382,9 @@ const NrCode::DeviceTables& NrCode::device_tables(int device) const {
     std::memcpy(h_shift + shift.size(), info.data(), info.size() * 4);
     uint32_t* h_rs = h_shift + shift.size() + info.size();
     for (size_t i = 0; i < row_start.size(); ++i) h_rs[i] = row_start[i];
-    // the columns, each with where the two checks before list the same variable (bits 16..21:
-    // the edge's position in the check before, bit 21 set when it is there; bits 22..27 the
-    // same for the check two before); the last position of a variable listed twice
+    // the columns, each with where the two checks before list the same variable: bits 16..20
+    // the edge's position in the check before and bit 21 set when it is there, bits 22..26
+    // and 27 the same for the check two before (the last position of a variable listed twice)
     uint32_t* h_col = h_rs + row_start.size();
     std::vector<int> pos1(cols_b, -1), pos2(cols_b, -1);  // positions in checks i - 1 and i - 2
     auto mark = [&](std::vector<int>& pos, uint32_t r, bool set) {
