@@ -286,3 +286,26 @@ def test_checks_listing_a_variable_twice():
     _, its32, post32 = code.decode_batch(torch.from_numpy(llr.astype(np.float32)).cuda(), want_post=True)
     assert np.array_equal(its32.cpu().numpy(), i32) and np.array_equal(post32.cpu().numpy(), p32)
     assert len(set(it_ref.tolist())) >= 3
+
+
+def test_layered_groups_of_16_bit_exact():
+    """Checks of 9 to 16 edges take a group of 16 lanes (the row mirror between the two
+    halves): the PEG (576, 288) checks joined in pairs (check 2j with check 2j + 1, a
+    variable both list kept once; 144 checks of 10 to 13 edges; Eb/N0 2.5 dB at rate 3/4), layered normalized min-sum
+    f64 and FP32 posteriors and iteration counts bit-identical to the oracle's."""
+    print("\n[rows] groups of 16, layered NMS", flush=True)
+    n, rows = read_alist(os.path.join(GOLDEN, "peg_n576_k288.alist"))
+    joined = [list(dict.fromkeys(list(rows[2 * j]) + list(rows[2 * j + 1]))) for j in range(len(rows) // 2)]
+    degs = [len(r) for r in joined]
+    assert 8 < min(degs) and max(degs) <= 16
+    code = F.NrLdpc.from_rows(n, joined).set_decoder("normalized_min_sum", 0.75, True, 20)
+    ref = O.Code.from_rows(n, joined)
+    llr = zero_codeword_llrs("groups16", n, 2.5, 0.75, 90, cond=8)
+    post_ref, it_ref = ref.decode_many(llr, rule=O.NMS, alpha=0.75, layered=True, max_iterations=20, threads=16)
+    _, its, post = code.decode_batch(torch.from_numpy(llr).cuda(), want_post=True)
+    assert np.array_equal(its.cpu().numpy(), it_ref) and np.array_equal(post.cpu().numpy(), post_ref)
+    p32, i32 = ref.decode_many_f32(llr.astype(np.float32), rule=O.NMS, alpha=0.75, layered=True,
+                                   max_iterations=20, threads=16)
+    _, its32, post32 = code.decode_batch(torch.from_numpy(llr.astype(np.float32)).cuda(), want_post=True)
+    assert np.array_equal(its32.cpu().numpy(), i32) and np.array_equal(post32.cpu().numpy(), p32)
+    assert len(set(it_ref.tolist())) >= 3
