@@ -13,6 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libcomfec_gpu.so")
+# the microbenchmarks of the HBM, LDS and VALU peaks (bench.py only; not part of the library)
+PEAKS_LIB = os.path.join(OUT_DIR, "libcomfec_peaks.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = ["nr_code.cpp", "capi.cpp", "sim.cpp", "sim_report.cpp", "campaign.cpp", "hooks_api.cpp", "learned_api.cpp", "batch.cpp", "decode_generic.hip", "decode_layered.hip", "decode_layered_f32.hip", "decode_layered_f16.hip", "trials.hip", "learned.hip"]
 FLAGS = [
@@ -56,11 +58,12 @@ def build(verbose=True):
     os.makedirs(OUT_DIR, exist_ok=True)
     defines = {"sim_report.cpp": ['-DCOMFEC_COMMIT="%s"' % commit()]}
     with ThreadPoolExecutor(max_workers=4) as ex:
-        objs = list(ex.map(lambda s: _compile(s, defines.get(s, ())), SOURCES))
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"]
-    subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
+        objs = list(ex.map(lambda s: _compile(s, defines.get(s, ())), SOURCES + ["peaks.hip"]))
+    for lib, parts in ((LIB, objs[:-1]), (PEAKS_LIB, objs[-1:])):
+        tmp = lib + ".tmp"
+        cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *parts, "-lpthread"]
+        subprocess.run(cmd, check=True)
+        os.replace(tmp, lib)
     if verbose:
         print("built", LIB)
     return LIB

@@ -142,10 +142,21 @@ SIGNATURES += [
     ("cf_hooks_json_logger", C.c_int, [C.c_void_p, C.c_char_p]),
     ("cf_hooks_destroy", None, [C.c_void_p]),
     ("cf_wilson", C.c_int, [C.c_uint64, C.c_uint64, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    ("cf_constellation_points", C.c_int, [C.c_int, _f64p, _f64p, C.c_size_t]),
 ]
 
 BPSK, QPSK, QAM16, QAM64 = 1, 2, 4, 6
 EBN0_DB, ESN0_DB = 0, 1
+
+
+def constellation_points(modulation):
+    """The unit-energy Gray QAM of the device generator (Constellation::new(Qam, M)
+    .normalized().points(), constellation.rs:168-205, 396-398) as complex128, point i the
+    symbol of label i."""
+    m = 1 << modulation
+    re, im = np.zeros(m), np.zeros(m)
+    _check(lib().cf_constellation_points(modulation, re, im, m))
+    return re + 1j * im
 
 _lib = None
 

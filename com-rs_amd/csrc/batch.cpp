@@ -57,7 +57,7 @@ int group_of(int z, int threads) { return std::max(1, std::min(threads / z, kMax
 
 // The layered kernel on `count` codewords of one code: a workgroup of the next multiple of
 // 64 lanes above Z takes floor(lanes / Z) codewords at a time.
-void run_layered(const cf_code* h, int device, const LayeredRun& r, hipStream_t stream) {
+void run_layered(const cf_code* h, int device, const LayeredRun& r, hipStream_t stream, uint32_t* counter) {
     const NrCode& c = h->code;
     if (!layered_supported(c.bg, int(c.z)))
         fail(Kind::Config, "the layered kernel needs a lifting size of at most " + std::to_string(kMaxThreads));
@@ -89,13 +89,20 @@ void run_layered(const cf_code* h, int device, const LayeredRun& r, hipStream_t 
     check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "device attribute");
     const int grid = int(std::min<int64_t>(a.n_tasks, int64_t(std::max(per_cu, 1)) * std::max(cus, 1)));
     LayeredLaunch l{c.bg, r.half, solo, zc, threads, std::max(grid, 1), lds};
+    if (counter) {
+        // the caller's counter: a fill kernel (no memset node) zeroes it before the launch
+        check_hip(launch_zero(reinterpret_cast<uint64_t*>(counter), 1, stream), "zero launch");
+        a.queue = counter;
+        check_hip(launch_layered(a, l, stream, true), "layered decode launch");
+        return;
+    }
     // the counter lives in the stream's workspace, held until both launches are queued
     WorkspaceSlot& slot = workspace(device, stream, kQueueSlot);
     std::lock_guard<std::mutex> lock(slot.mu);
-    uint64_t* counter = static_cast<uint64_t*>(slot.ws.get(sizeof(uint64_t)));
-    if (!counter) fail(Kind::Device, "task counter: out of device memory");
-    check_hip(launch_zero(counter, 1, stream), "zero launch");
-    a.queue = reinterpret_cast<uint32_t*>(counter);
+    uint64_t* ws_counter = static_cast<uint64_t*>(slot.ws.get(sizeof(uint64_t)));
+    if (!ws_counter) fail(Kind::Device, "task counter: out of device memory");
+    check_hip(launch_zero(ws_counter, 1, stream), "zero launch");
+    a.queue = reinterpret_cast<uint32_t*>(ws_counter);
     check_hip(launch_layered(a, l, stream, true), "layered decode launch");
 }
 

@@ -21,11 +21,13 @@ using namespace comfec::detail;
 
 namespace {
 
-// Gray square QAM of 2^bps points, unit average energy (constellation.rs:168-205, 396-398).
-void qam_points(int bps, float* re, float* im) {
+// Gray square QAM of 2^bps points, unit average energy (constellation.rs:168-205, 396-398):
+// the natural order walks the rows from the top, left to right on even rows and right to
+// left on odd ones, and point gray(i) of the labelled constellation is natural point i.
+void qam_points(int bps, double* re, double* im) {
     const int m = 1 << bps;
     const int l = int(std::lround(std::sqrt(double(m))));
-    std::vector<double> nr(m), ni(m), pr(m), pi(m);
+    std::vector<double> nr(m), ni(m);
     int p = 0;
     for (int row = 0; row < l; ++row) {
         const double y = 2.0 * double(l - 1 - row) - (double(l) - 1.0);
@@ -37,15 +39,15 @@ void qam_points(int bps, float* re, float* im) {
         }
     }
     for (int i = 0; i < m; ++i) {
-        pr[i ^ (i >> 1)] = nr[i];
-        pi[i ^ (i >> 1)] = ni[i];
+        re[i ^ (i >> 1)] = nr[i];
+        im[i ^ (i >> 1)] = ni[i];
     }
     double energy = 0.0;
-    for (int i = 0; i < m; ++i) energy += pr[i] * pr[i] + pi[i] * pi[i];
+    for (int i = 0; i < m; ++i) energy += re[i] * re[i] + im[i] * im[i];
     const double s = 1.0 / std::sqrt(energy / double(m));
     for (int i = 0; i < m; ++i) {
-        re[i] = float(pr[i] * s);
-        im[i] = float(pi[i] * s);
+        re[i] *= s;
+        im[i] *= s;
     }
 }
 
@@ -92,7 +94,12 @@ TrialGenArgs make_generator_args(const NrCode& c, int device, int modulation, in
         a.noise_var = float(n0);
         a.noise_sd = float(std::sqrt(n0 / 2.0));
         a.log_prior = float(-std::log(double(1 << modulation)));
-        qam_points(modulation, a.pts_re, a.pts_im);
+        double re[64], im[64];
+        qam_points(modulation, re, im);
+        for (int i = 0; i < (1 << modulation); ++i) {
+            a.pts_re[i] = float(re[i]);
+            a.pts_im[i] = float(im[i]);
+        }
     }
     a.k0 = uint32_t(seed);
     a.k1 = uint32_t(seed >> 32);
@@ -289,6 +296,17 @@ class NrLinkScenario : public sim::BatchScenario {
 }  // namespace
 
 extern "C" {
+
+cf_status cf_constellation_points(int modulation, double* re_host, double* im_host, size_t m) {
+    return guarded([&] {
+        if (modulation != CF_QPSK && modulation != CF_QAM16 && modulation != CF_QAM64)
+            fail(Kind::Config, "the constellation is QPSK, 16-QAM or 64-QAM, got " + std::to_string(modulation) +
+                                   " bits per symbol");
+        expect_len(m, size_t(1) << modulation, "points");
+        if (!re_host || !im_host) fail(Kind::Config, "null output buffer");
+        qam_points(modulation, re_host, im_host);
+    });
+}
 
 cf_status cf_generate_batch(const cf_code* h, int modulation, int snr_kind, double snr_db, uint64_t seed,
                             uint32_t point, uint64_t trial0, size_t count, float* d_llr, uint32_t* d_info,

@@ -95,24 +95,69 @@ bool fast_layered(const cf_code* h) {
     return s.layered && (s.rule == 1 || s.rule == 2) && layered_supported(h->code.bg, int(h->code.z));
 }
 
+// What one host-buffer call needs on a device: its own stream (so that calls from several
+// threads run side by side and wait only for their own work) and a grow-only buffer for the
+// codeword's copies. Contexts are kept in a pool for the life of the process and handed to
+// one call at a time, so that after the first calls no call allocates device memory.
+struct HostCtx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Workspace io;
+};
+
+std::mutex g_ctx_mu;
+std::vector<HostCtx*> g_ctx_free;
+
+class HostCtxLease {
+public:
+    explicit HostCtxLease(int device) {
+        {
+            std::lock_guard<std::mutex> l(g_ctx_mu);
+            for (size_t i = 0; i < g_ctx_free.size(); ++i)
+                if (g_ctx_free[i]->device == device) {
+                    ctx_ = g_ctx_free[i];
+                    g_ctx_free.erase(g_ctx_free.begin() + long(i));
+                    break;
+                }
+        }
+        if (!ctx_) {
+            auto c = std::make_unique<HostCtx>();
+            c->device = device;
+            check_hip(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+            ctx_ = c.release();
+        }
+    }
+    ~HostCtxLease() {
+        std::lock_guard<std::mutex> l(g_ctx_mu);
+        g_ctx_free.push_back(ctx_);
+    }
+    HostCtx& operator*() const { return *ctx_; }
+    HostCtx* operator->() const { return ctx_; }
+
+private:
+    HostCtx* ctx_ = nullptr;
+};
+
 // One codeword from host f64 buffers, decoded on the GPU in f64.
 void decode_host_f64(const cf_code* h, const double* llr, bool full, const double* apriori, double* post_host,
                      uint8_t* bits_host, uint32_t* iterations) {
     const NrCode& c = h->code;
     int device = current_device();
-    hipStream_t stream = nullptr;
+    HostCtxLease ctx(device);
+    hipStream_t stream = ctx->stream;
     size_t in_len = full ? c.n : c.e;
     size_t bytes = (in_len + c.k_info + c.n) * sizeof(double) + c.k_info + sizeof(uint32_t) + 64;
-    void* buf = nullptr;
-    check_hip(hipMalloc(&buf, bytes), "hipMalloc");
-    std::unique_ptr<void, void (*)(void*)> guard(buf, [](void* p) { (void)hipFree(p); });
+    void* buf = ctx->io.get(bytes);
+    if (!buf) fail(Kind::Device, "out of device memory for a codeword of " + std::to_string(in_len) + " LLRs");
     double* d_llr = static_cast<double*>(buf);
     double* d_apriori = d_llr + in_len;
     double* d_post = d_apriori + c.k_info;
     uint32_t* d_it = reinterpret_cast<uint32_t*>(d_post + c.n);
     uint8_t* d_bits = reinterpret_cast<uint8_t*>(d_it + 1);
-    check_hip(hipMemcpy(d_llr, llr, in_len * sizeof(double), hipMemcpyHostToDevice), "copy in");
-    if (apriori) check_hip(hipMemcpy(d_apriori, apriori, c.k_info * sizeof(double), hipMemcpyHostToDevice), "copy in");
+    check_hip(hipMemcpyAsync(d_llr, llr, in_len * sizeof(double), hipMemcpyHostToDevice, stream), "copy in");
+    if (apriori)
+        check_hip(hipMemcpyAsync(d_apriori, apriori, c.k_info * sizeof(double), hipMemcpyHostToDevice, stream),
+                  "copy in");
     GenericArgs<double> a = generic_args<double>(h, device);
     a.input_full = full ? 1 : 0;
     a.llr = d_llr;
@@ -122,12 +167,74 @@ void decode_host_f64(const cf_code* h, const double* llr, bool full, const doubl
     a.post_out = d_post;
     a.iterations = d_it;
     run_generic(a, device, stream);
-    check_hip(hipStreamSynchronize(stream), "decode");
     uint32_t it = 0;
-    check_hip(hipMemcpy(&it, d_it, sizeof it, hipMemcpyDeviceToHost), "copy out");
+    check_hip(hipMemcpyAsync(&it, d_it, sizeof it, hipMemcpyDeviceToHost, stream), "copy out");
+    if (post_host)
+        check_hip(hipMemcpyAsync(post_host, d_post, c.n * sizeof(double), hipMemcpyDeviceToHost, stream), "copy out");
+    if (bits_host) check_hip(hipMemcpyAsync(bits_host, d_bits, c.k_info, hipMemcpyDeviceToHost, stream), "copy out");
+    check_hip(hipStreamSynchronize(stream), "decode");
     if (iterations) *iterations = it;
-    if (post_host) check_hip(hipMemcpy(post_host, d_post, c.n * sizeof(double), hipMemcpyDeviceToHost), "copy out");
-    if (bits_host) check_hip(hipMemcpy(bits_host, d_bits, c.k_info, hipMemcpyDeviceToHost), "copy out");
+}
+
+// cf_decode_batch, with its scratch (the layered kernel's task counter, the general kernel's
+// messages) taken from the workspace of the stream, or, when `fixed` is given, from that
+// buffer alone: then nothing is allocated, and the code's tables must already be on the device.
+void decode_batch(const cf_code* h, int msg_type, const void* d_llr, size_t count, uint8_t* d_bits, void* d_post,
+                  uint32_t* d_iterations, hipStream_t s, const FixedScratch* fixed) {
+    if (!h) fail(Kind::Config, "null handle");
+    if (count == 0) return;
+    if (!d_llr) fail(Kind::Config, "null LLR buffer");
+    int device = current_device();
+    if (fixed && !h->code.tables_resident(device))
+        fail(Kind::Config, "the code's tables are not on device " + std::to_string(device) +
+                               " yet (an entry point that allocates nothing cannot upload them): make one decode "
+                               "or generator call with this code on that device first");
+    auto generic = [&](auto a) {
+        if (!fixed) {
+            run_generic(a, device, s);
+            return;
+        }
+        Workspace ws;
+        ws.ptr = fixed->messages;
+        ws.bytes = fixed->message_bytes;
+        ws.fixed = true;
+        const hipError_t err = launch_generic(a, device, s, ws);
+        if (err == hipErrorOutOfMemory)
+            fail(Kind::Config, "a codeword of this code needs more message scratch than the " +
+                                   std::to_string(fixed->message_bytes >> 20) + " MiB the library holds");
+        check_hip(err, "decode launch");
+    };
+    if (msg_type == CF_MSG_F64) {
+        GenericArgs<double> a = generic_args<double>(h, device);
+        a.llr = static_cast<const double*>(d_llr);
+        a.count = int64_t(count);
+        a.bits = d_bits;
+        a.post_out = static_cast<double*>(d_post);
+        a.iterations = d_iterations;
+        generic(a);
+    } else if ((msg_type == CF_MSG_F32 || msg_type == CF_MSG_F16) && fast_layered(h)) {
+        LayeredRun r;
+        r.llr = static_cast<const float*>(d_llr);
+        r.count = int64_t(count);
+        r.bits = d_bits;
+        r.post = static_cast<float*>(d_post);
+        r.iterations = d_iterations;
+        r.half = msg_type == CF_MSG_F16;
+        run_layered(h, device, r, s, fixed ? fixed->counter : nullptr);
+    } else if (msg_type == CF_MSG_F16) {
+        fail(Kind::Config, "FP16 storage runs the layered normalized or offset min-sum");
+    } else if (msg_type == CF_MSG_F32) {
+        GenericArgs<float> a = generic_args<float>(h, device);
+        a.llr = static_cast<const float*>(d_llr);
+        a.count = int64_t(count);
+        a.bits = d_bits;
+        a.post_out = static_cast<float*>(d_post);
+        a.iterations = d_iterations;
+        generic(a);
+    } else {
+        fail(Kind::Config, "message type must be CF_MSG_F64, CF_MSG_F32 or CF_MSG_F16, got " +
+                               std::to_string(msg_type));
+    }
 }
 
 }  // namespace detail
@@ -231,42 +338,8 @@ cf_status cf_app_into(const cf_code* h, const double* llr, size_t e, const doubl
 cf_status cf_decode_batch(const cf_code* h, int msg_type, const void* d_llr, size_t count, uint8_t* d_bits,
                           void* d_post, uint32_t* d_iterations, void* stream) {
     return guarded([&] {
-        if (!h) fail(Kind::Config, "null handle");
-        if (count == 0) return;
-        if (!d_llr) fail(Kind::Config, "null LLR buffer");
-        int device = current_device();
-        hipStream_t s = static_cast<hipStream_t>(stream);
-        if (msg_type == CF_MSG_F64) {
-            GenericArgs<double> a = generic_args<double>(h, device);
-            a.llr = static_cast<const double*>(d_llr);
-            a.count = int64_t(count);
-            a.bits = d_bits;
-            a.post_out = static_cast<double*>(d_post);
-            a.iterations = d_iterations;
-            run_generic(a, device, s);
-        } else if ((msg_type == CF_MSG_F32 || msg_type == CF_MSG_F16) && fast_layered(h)) {
-            LayeredRun r;
-            r.llr = static_cast<const float*>(d_llr);
-            r.count = int64_t(count);
-            r.bits = d_bits;
-            r.post = static_cast<float*>(d_post);
-            r.iterations = d_iterations;
-            r.half = msg_type == CF_MSG_F16;
-            run_layered(h, device, r, s);
-        } else if (msg_type == CF_MSG_F16) {
-            fail(Kind::Config, "FP16 storage runs the layered normalized or offset min-sum");
-        } else if (msg_type == CF_MSG_F32) {
-            GenericArgs<float> a = generic_args<float>(h, device);
-            a.llr = static_cast<const float*>(d_llr);
-            a.count = int64_t(count);
-            a.bits = d_bits;
-            a.post_out = static_cast<float*>(d_post);
-            a.iterations = d_iterations;
-            run_generic(a, device, s);
-        } else {
-            fail(Kind::Config, "message type must be CF_MSG_F64, CF_MSG_F32 or CF_MSG_F16, got " +
-                                   std::to_string(msg_type));
-        }
+        decode_batch(h, msg_type, d_llr, count, d_bits, d_post, d_iterations, static_cast<hipStream_t>(stream),
+                     nullptr);
     });
 }
 

@@ -76,7 +76,18 @@ struct LayeredRun {
     bool fallback = false;
     bool half = false;
 };
-void run_layered(const cf_code* h, int device, const LayeredRun& r, hipStream_t stream);
+// counter: the task counter of the launch (device memory), or null for the stream's workspace
+void run_layered(const cf_code* h, int device, const LayeredRun& r, hipStream_t stream, uint32_t* counter = nullptr);
+// Scratch given whole to a decode: the layered kernel's task counter and the general
+// kernels' message buffer (device memory of the current device).
+struct FixedScratch {
+    uint32_t* counter;
+    void* messages;
+    size_t message_bytes;
+};
+// cf_decode_batch; with `fixed`, no allocation (its scratch only, the code's tables resident)
+void decode_batch(const cf_code* h, int msg_type, const void* d_llr, size_t count, uint8_t* d_bits, void* d_post,
+                  uint32_t* d_iterations, hipStream_t s, const FixedScratch* fixed);
 BatchClass batch_class(const NrCode& c, const DecoderSettings& s, int device, int g);
 
 // Runs f, mapping a thrown Error to its status and the thread's last message.

@@ -733,6 +733,11 @@ hipError_t launch_rows(const GenericArgs<T>& a0, Workspace& ws, hipStream_t stre
     const size_t per_lane = (size_t(c.n) + size_t(c.edges_b)) * sizeof(T);
     int64_t lanes = int64_t(kRowsScratchBytes / per_lane) & ~int64_t(63);
     if (lanes < 64) lanes = 64;
+    if (ws.fixed) {
+        // a buffer given whole: as many lanes as it holds
+        if (ws.bytes < per_lane) return hipErrorOutOfMemory;
+        lanes = std::min<int64_t>(lanes, int64_t(ws.bytes / per_lane));
+    }
     if (lanes > a.count) lanes = a.count;
     if (lanes < 1) return hipSuccess;
     // fewer lanes (each then decodes more codewords) when the device cannot hold them all
@@ -758,6 +763,10 @@ hipError_t launch_groups(const GenericArgs<T>& a0, Workspace& ws, hipStream_t st
     // posteriors, messages and one value the lanes past a check's edges write
     const size_t per_slot = (size_t(c.n) + size_t(c.edges_b) + 1) * sizeof(T);
     int64_t slots = int64_t(kRowsScratchBytes / per_slot);
+    if (ws.fixed) {
+        if (ws.bytes < per_slot) return hipErrorOutOfMemory;
+        slots = std::min<int64_t>(slots, int64_t(ws.bytes / per_slot));
+    }
     if (slots > a.count) slots = a.count;
     if (slots < 1) return hipSuccess;
     // fewer groups (each then decodes more codewords) when the device cannot hold them all
@@ -808,6 +817,12 @@ hipError_t launch(const GenericArgs<T>& a0, int device, hipStream_t stream, Work
     if (grid < 1) return hipSuccess;
     a.scratch_stride = size_t(c.edges_b) * c.z + (post_lds ? 0 : c.n) + (a.layered ? 0 : c.n);
     a.scratch_stride = (a.scratch_stride + 63) & ~size_t(63);
+    if (ws.fixed) {
+        // a buffer given whole: as many workgroups as it holds
+        const size_t per = a.scratch_stride * sizeof(T);
+        if (ws.bytes < per) return hipErrorOutOfMemory;
+        grid = std::min<int64_t>(grid, int64_t(ws.bytes / per));
+    }
     T* scratch = static_cast<T*>(ws.get(size_t(grid) * a.scratch_stride * sizeof(T)));
     if (!scratch) return hipErrorOutOfMemory;
     a.scratch = scratch;

@@ -177,16 +177,26 @@ hipError_t launch_count(const CountArgs& a, hipStream_t stream);
 // a fill kernel (not a memset node, so that graph replays clear the counters too)
 hipError_t launch_zero(uint64_t* p, int n, hipStream_t stream);
 
-// A grow-only device buffer, one per (device, stream) in the C ABI layer.
+// A grow-only device buffer, one per (device, stream) in the C ABI layer; or, with `fixed`
+// set, a buffer of `bytes` given whole that is never reallocated (the launchers then size
+// their persistent grids to it).
 struct Workspace {
     void* ptr = nullptr;
     size_t bytes = 0;
+    bool fixed = false;
     void* get(size_t want) {
         if (want <= bytes) return ptr;
+        if (fixed) return nullptr;
         if (ptr) (void)hipFree(ptr);
         ptr = nullptr;
         bytes = 0;
-        if (hipMalloc(&ptr, want) != hipSuccess) return nullptr;
+        if (hipMalloc(&ptr, want) != hipSuccess) {
+            // the failure stays the thread's last error: clear it, so that a later launch
+            // check (hipGetLastError) reports that launch and not this allocation
+            (void)hipGetLastError();
+            ptr = nullptr;
+            return nullptr;
+        }
         bytes = want;
         return ptr;
     }

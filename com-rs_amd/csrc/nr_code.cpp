@@ -349,6 +349,11 @@ void NrCode::encode(const uint8_t* bits, uint8_t* out) const {
     for (uint32_t t = 0; t < e; ++t) out[t] = cw[map[t]];
 }
 
+bool NrCode::tables_resident(int device) const {
+    std::lock_guard<std::mutex> lock(dev_mu_);
+    return device >= 0 && size_t(device) < dev_.size() && dev_[device].base != nullptr;
+}
+
 const NrCode::DeviceTables& NrCode::device_tables(int device) const {
     std::lock_guard<std::mutex> lock(dev_mu_);
     if (dev_.size() <= size_t(device)) dev_.resize(device + 1);

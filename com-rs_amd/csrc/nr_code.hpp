@@ -97,6 +97,8 @@ struct NrCode {
         const uint32_t* col32 = nullptr;
     };
     const DeviceTables& device_tables(int device) const;
+    // whether the tables are already on `device` (then device_tables allocates nothing)
+    bool tables_resident(int device) const;
     ~NrCode();
     NrCode() = default;
     NrCode(NrCode&&) noexcept;

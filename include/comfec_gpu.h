@@ -185,6 +185,12 @@ cf_status cf_decode_batch_uneven(const cf_code* const* codes, const uint64_t* ll
 /* Modulations of the generator: BPSK on a real channel, or Gray square QAM (QPSK, 16-QAM,
    64-QAM) normalized to unit energy (crates/com-mapping/src/constellation.rs:168-205). */
 enum { CF_BPSK = 1, CF_QPSK = 2, CF_QAM16 = 4, CF_QAM64 = 6 };
+/* Constellation::new(Qam, 2^modulation).normalized().points() (crates/com-mapping/src/
+   constellation.rs:168-205, 396-398): the m = 2^modulation points of the generator's Gray
+   QAM (modulation CF_QPSK, CF_QAM16 or CF_QAM64), point i the symbol of label i, its bits
+   read most significant first. */
+cf_status cf_constellation_points(int modulation, double* re_host, double* im_host, size_t m);
+
 /* How the noise is given: Eb/N0 in dB at the code rate K'/E, or Es/N0 in dB. */
 enum { CF_EBN0_DB = 0, CF_ESN0_DB = 1 };
 

@@ -313,6 +313,101 @@ static float or_spa_phi(float x) {
 #undef TANH
 #undef ATANH
 
+/* The layered min-sum (NMS or OMS) of ldpc.rs:509-531,546-557 run in f64 and in FP32 side by
+   side on the same channel LLRs (values exactly representable in FP32), each with its own
+   syndrome stop, with a record of where the two can part: the smallest f64 magnitude of a
+   check-node input or a posterior over the f64 run (a tie below it is a recorded tie), the
+   first point where an FP32 sign differs from the f64 one (its f64 magnitude and
+   iteration), the rows whose FP32 minimum is at another edge while the f64 minimum is
+   strict (the smallest relative f64 gap m2 / m1 - 1 among them), and per iteration the
+   largest |post32 - post64| / max(|post64|, 1). Layout of `out` (10 + max_iterations):
+   it64, it32, min_abs, first_sign_abs, first_sign_it, sign_events, argmin_events,
+   min_argmin_gap, first_argmin_it, decisions_differ, rel[1..max_iterations]. */
+int or_trace_f32_f64(const or_code* c, const or_dec* d, const double* llr, double* out) {
+    if (d->rule != OR_NMS && d->rule != OR_OMS) return -1;
+    const int T = d->max_iterations;
+    int32_t edges = c->row_start[c->m], n = c->n;
+    double* p64 = (double*)malloc(sizeof(double) * (size_t)n);
+    float* p32 = (float*)malloc(sizeof(float) * (size_t)n);
+    double* c64 = (double*)calloc((size_t)edges, sizeof(double));
+    float* c32 = (float*)calloc((size_t)edges, sizeof(float));
+    int dm = 0;
+    for (int32_t r = 0; r < c->m; ++r) {
+        int deg = c->row_start[r + 1] - c->row_start[r];
+        if (deg > dm) dm = deg;
+    }
+    double *in64 = (double*)malloc(sizeof(double) * (dm + 1)), *s64 = (double*)malloc(sizeof(double) * (dm + 1));
+    float *in32 = (float*)malloc(sizeof(float) * (dm + 1)), *s32 = (float*)malloc(sizeof(float) * (dm + 1));
+    for (int32_t v = 0; v < n; ++v) {
+        p64[v] = llr[v];
+        p32[v] = (float)llr[v];
+    }
+    for (int i = 0; i < 10 + T; ++i) out[i] = 0.0;
+    double min_abs = INFINITY, first_sign = -1.0, min_gap = INFINITY;
+    int first_sign_it = 0, first_argmin_it = 0, sign_events = 0, argmin_events = 0;
+    for (int32_t v = 0; v < n; ++v) min_abs = fmin(min_abs, fabs(p64[v]));
+    int done64 = is_codeword_post(c, p64), done32 = is_codeword_post_f32(c, p32);
+    int it64 = done64 ? 0 : T, it32 = done32 ? 0 : T;
+    for (int it = 1; it <= T && !(done64 && done32); ++it) {
+        for (int32_t r = 0; r < c->m; ++r) {
+            int32_t s = c->row_start[r];
+            int deg = c->row_start[r + 1] - s;
+            if (!done64) {
+                for (int i = 0; i < deg; ++i) in64[i] = p64[c->cols[s + i]] - c64[s + i];
+                check_message(d, in64, deg, c64 + s, s64);
+                for (int i = 0; i < deg; ++i) p64[c->cols[s + i]] = in64[i] + c64[s + i];
+            }
+            if (!done32) {
+                for (int i = 0; i < deg; ++i) in32[i] = p32[c->cols[s + i]] - c32[s + i];
+                check_message_f32(d, in32, deg, c32 + s, s32);
+                for (int i = 0; i < deg; ++i) p32[c->cols[s + i]] = in32[i] + c32[s + i];
+            }
+            if (done64 || done32) continue;
+            int a64 = 0, a32 = 0;
+            double m1 = INFINITY, m2 = INFINITY;
+            float f1 = INFINITY;
+            for (int i = 0; i < deg; ++i) {
+                double x = in64[i], post = p64[c->cols[s + i]];
+                min_abs = fmin(min_abs, fmin(fabs(x), fabs(post)));
+                int flip = ((x < 0.0) != (in32[i] < 0.0f)) || ((post < 0.0) != (p32[c->cols[s + i]] < 0.0f));
+                if (flip) {
+                    if (sign_events == 0) {
+                        first_sign = ((x < 0.0) != (in32[i] < 0.0f)) ? fabs(x) : fabs(post);
+                        first_sign_it = it;
+                    }
+                    ++sign_events;
+                }
+                if (fabs(x) < m1) { m2 = m1; m1 = fabs(x); a64 = i; } else if (fabs(x) < m2) m2 = fabs(x);
+                if (fabsf(in32[i]) < f1) { f1 = fabsf(in32[i]); a32 = i; }
+            }
+            if (a64 != a32 && m2 > m1) {
+                if (argmin_events == 0) first_argmin_it = it;
+                ++argmin_events;
+                min_gap = fmin(min_gap, m1 > 0.0 ? m2 / m1 - 1.0 : INFINITY);
+            }
+        }
+        if (!done64 && is_codeword_post(c, p64)) { done64 = 1; it64 = it; }
+        if (!done32 && is_codeword_post_f32(c, p32)) { done32 = 1; it32 = it; }
+        double rel = 0.0;
+        for (int32_t v = 0; v < n; ++v) rel = fmax(rel, fabs((double)p32[v] - p64[v]) / fmax(fabs(p64[v]), 1.0));
+        out[10 + it - 1] = rel;
+    }
+    int diff = 0;
+    for (int32_t v = 0; v < n; ++v) diff += (p64[v] < 0.0) != (p32[v] < 0.0f);
+    out[0] = it64;
+    out[1] = it32;
+    out[2] = min_abs;
+    out[3] = first_sign;
+    out[4] = first_sign_it;
+    out[5] = sign_events;
+    out[6] = argmin_events;
+    out[7] = min_gap;
+    out[8] = first_argmin_it;
+    out[9] = diff;
+    free(p64); free(p32); free(c64); free(c32); free(in64); free(s64); free(in32); free(s32);
+    return 0;
+}
+
 int or_nr_decode_block(const or_code* c, const or_dec* d, const double* llr, uint8_t* bits) {
     double* full = (double*)malloc(sizeof(double) * c->n);
     double* post = (double*)malloc(sizeof(double) * c->n);

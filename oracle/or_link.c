@@ -5,6 +5,7 @@
 #include <math.h>
 #include <string.h>
 #include <stdlib.h>
+#include <pthread.h>
 #include "oracle.h"
 
 /* crates/com-core/src/rng.rs:49-55 */
@@ -144,7 +145,28 @@ void or_qam_points(int m, double* re, double* im) {
     free(nat_im);
 }
 
-/* demapper.rs:270-297 with LabelHalves::new/llr in max-log (demapper.rs:94-146) */
+/* LabelHalves::new and LabelHalves::llr with Jacobian::MaxLog (demapper.rs:94-146): the
+   largest metric of the labels whose bit `shift` is 0, minus the largest of those where it
+   is 1 (f64::max, which ignores a NaN like fmax) */
+double or_label_halves_llr(const double* metric, int m, int shift) {
+    double m0 = -INFINITY, m1 = -INFINITY;
+    for (int i = 0; i < m; ++i) {
+        if (((i >> shift) & 1) == 0) m0 = fmax(m0, metric[i]);
+        else m1 = fmax(m1, metric[i]);
+    }
+    return m0 - m1;
+}
+
+/* llr_from_metrics (demapper.rs:130-137), max-log: b = log2 m LLRs, the most significant
+   bit first */
+void or_llr_from_metrics(const double* metric, int m, double* out) {
+    int b = 0;
+    while ((1 << b) < m) ++b;
+    for (int o = 0; o < b; ++o) out[o] = or_label_halves_llr(metric, m, b - 1 - o);
+}
+
+/* demapper.rs:270-297 (llr_kernel with the uniform prior -ln M added to every metric before
+   the maxima) with llr_from_metrics in max-log */
 void or_qam_llr(int m, const double* yre, const double* yim, int64_t nsym, double noise_var,
                 double* llr) {
     double pre[256], pim[256], metric[256];
@@ -157,15 +179,7 @@ void or_qam_llr(int m, const double* yre, const double* yim, int64_t nsym, doubl
             double dr = yre[n] - pre[i], di = yim[n] - pim[i];
             metric[i] = -(dr * dr + di * di) / noise_var + uniform;
         }
-        for (int o = 0; o < b; ++o) {
-            int shift = b - 1 - o;
-            double m0 = -INFINITY, m1 = -INFINITY;
-            for (int i = 0; i < m; ++i) {
-                if (((i >> shift) & 1) == 0) m0 = fmax(m0, metric[i]);
-                else m1 = fmax(m1, metric[i]);
-            }
-            llr[n * b + o] = m0 - m1;
-        }
+        or_llr_from_metrics(metric, m, llr + n * b);
     }
 }
 
@@ -328,4 +342,108 @@ void or_philox_info_bits(uint64_t seed, uint32_t point, uint64_t trial, int32_t 
                 if (p < k_info) bits[p] = (uint8_t)((out[q] >> b) & 1u);
             }
     }
+}
+
+/* The channel noise of the device trial generator (no reference counterpart), restated in
+   f64: draw `index` of purpose 1 of the trial is Philox(counter = (index, trial low, trial
+   high, point*16 + 1), seed); u1, u2 are its first two words as (x >> 8) + 1 over 2^24 (on
+   (0, 1]), and Box-Muller gives g0 = sqrt(-2 ln u1) cos(2 pi u2), g1 = ... sin(2 pi u2). */
+void or_philox_normals(uint64_t seed, uint32_t point, uint64_t trial, uint32_t index, double* g0,
+                       double* g1) {
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t ctr[4] = {index, (uint32_t)trial, (uint32_t)(trial >> 32), (point << 4) | 1u};
+    uint32_t out[4];
+    or_philox4x32_10(ctr, key, out);
+    double u1 = (double)((out[0] >> 8) + 1u) / 16777216.0;
+    double u2 = (double)((out[1] >> 8) + 1u) / 16777216.0;
+    double rad = sqrt(-2.0 * log(u1));
+    *g0 = rad * cos(2.0 * OR_PI * u2);
+    *g1 = rad * sin(2.0 * OR_PI * u2);
+}
+
+/* One trial of the device trial generator on an NR code, in f64: the Philox information bits,
+   NrLdpc::encode_block (nr/ldpc.rs:208-214), then for bps == 1 BPSK y = (1 - 2c) + sd g0 with
+   sd = sqrt(noise_var) and LLR 2y/noise_var (validation/src/lib.rs:67-71), or Gray QAM of 2^bps
+   points with the labels taken MSB first from consecutive transmitted bits (bits.rs:68-70),
+   y = p + sqrt(noise_var / 2) (g0 + j g1) (noise_var = N0 per symbol, awgn.rs:94-99) and the
+   max-log LLRs of demapper.rs:270-297. yre, yim (may be NULL) get the received symbols. */
+void or_philox_trial(const or_code* c, int bps, double noise_var, uint64_t seed, uint32_t point,
+                     uint64_t trial, uint8_t* info, double* yre, double* yim, double* llr) {
+    or_philox_info_bits(seed, point, trial, (int32_t)or_code_info(c, 3), info);
+    int32_t e = (int32_t)or_code_info(c, 4);
+    uint8_t* tx = (uint8_t*)malloc((size_t)e);
+    or_nr_encode(c, info, tx);
+    int32_t nsym = e / bps;
+    if (bps == 1) {
+        double sd = sqrt(noise_var);
+        for (int32_t s = 0; s < nsym; ++s) {
+            double g0, g1;
+            or_philox_normals(seed, point, trial, (uint32_t)s, &g0, &g1);
+            double y = (1.0 - 2.0 * (double)tx[s]) + sd * g0;
+            if (yre) yre[s] = y;
+            if (yim) yim[s] = 0.0;
+            llr[s] = 2.0 * y / noise_var;
+        }
+    } else {
+        int m = 1 << bps;
+        double pre[256], pim[256];
+        or_qam_points(m, pre, pim);
+        double sd = sqrt(noise_var / 2.0);
+        double* yr = (double*)malloc(sizeof(double) * (size_t)nsym);
+        double* yi = (double*)malloc(sizeof(double) * (size_t)nsym);
+        for (int32_t s = 0; s < nsym; ++s) {
+            int label = 0;
+            for (int l = 0; l < bps; ++l) label = (label << 1) | (tx[s * bps + l] & 1);
+            double g0, g1;
+            or_philox_normals(seed, point, trial, (uint32_t)s, &g0, &g1);
+            yr[s] = pre[label] + sd * g0;
+            yi[s] = pim[label] + sd * g1;
+            if (yre) yre[s] = yr[s];
+            if (yim) yim[s] = yi[s];
+        }
+        or_qam_llr(m, yr, yi, nsym, noise_var, llr);
+        free(yr);
+        free(yi);
+    }
+    free(tx);
+}
+
+/* or_trial_nr_qam for trials trial0 .. trial0 + count - 1 on `threads` POSIX threads: the
+   information bits count x k_info, the LLRs count x e. */
+typedef struct {
+    const or_code* c;
+    int m;
+    double noise_var;
+    uint64_t seed;
+    const char* name;
+    uint64_t cond, trial0;
+    int64_t lo, hi;
+    uint8_t* info;
+    double* llr;
+} qam_job;
+
+static void* qam_worker(void* p) {
+    qam_job* j = (qam_job*)p;
+    int64_t k = or_code_info(j->c, 3), e = or_code_info(j->c, 4);
+    for (int64_t t = j->lo; t < j->hi; ++t)
+        or_trial_nr_qam(j->c, j->m, j->noise_var, j->seed, j->name, j->cond, j->trial0 + (uint64_t)t,
+                        j->info + t * k, j->llr + t * e);
+    return NULL;
+}
+
+void or_trial_nr_qam_many(const or_code* c, int m, double noise_var, uint64_t seed, const char* name,
+                          uint64_t cond, uint64_t trial0, int64_t count, uint8_t* info, double* llr,
+                          int threads) {
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t th[256];
+    qam_job jobs[256];
+    for (int t = 0; t < threads; ++t) {
+        jobs[t] = (qam_job){c, m, noise_var, seed, name, cond, trial0, count * t / threads,
+                            count * (t + 1) / threads, info, llr};
+        if (threads == 1) qam_worker(&jobs[t]);
+        else pthread_create(&th[t], NULL, qam_worker, &jobs[t]);
+    }
+    if (threads > 1)
+        for (int t = 0; t < threads; ++t) pthread_join(th[t], NULL);
 }

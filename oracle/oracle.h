@@ -70,6 +70,10 @@ int or_decode_llr_f32(const or_code* c, const or_dec* d, const float* llr, float
 void or_decode_many_f32(const or_code* c, const or_dec* d, const float* llr, int64_t count,
                         float* post, int32_t* iterations, int threads);
 
+/* the layered min-sum in f64 and in FP32 side by side, with a record of where they part
+   (see or_code.c): out holds 10 + d->max_iterations values; -1 for a rule other than NMS/OMS */
+int or_trace_f32_f64(const or_code* c, const or_dec* d, const double* llr, double* out);
+
 /* the layered min-sum (rules NMS and OMS) with the posteriors and check-to-variable
    messages stored in IEEE binary16 and the arithmetic in FP32 (the device FP16 mode);
    max_iterations, when not null, overrides d->max_iterations per codeword */
@@ -101,10 +105,20 @@ void or_qam_awgn_llr(int m, const uint8_t* bits, int64_t nbits, double noise_var
                      double* llr);
 void or_qam_llr(int m, const double* yre, const double* yim, int64_t nsym, double noise_var,
                 double* llr);
+/* LabelHalves::new(metric, shift, MaxLog).llr() (demapper.rs:94-146) and llr_from_metrics
+   (demapper.rs:130-137) in max-log, the most significant bit first */
+double or_label_halves_llr(const double* metric, int m, int shift);
+void or_llr_from_metrics(const double* metric, int m, double* out);
 
 /* Philox-4x32-10 and the information bits of the device trial generator */
 void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void or_philox_info_bits(uint64_t seed, uint32_t point, uint64_t trial, int32_t k_info, uint8_t* bits);
+/* its Box-Muller normals of draw `index` (purpose 1), and one whole trial of it in f64:
+   information bits, encoder, BPSK (bps 1) or Gray QAM (bps 2, 4, 6), AWGN, LLRs */
+void or_philox_normals(uint64_t seed, uint32_t point, uint64_t trial, uint32_t index, double* g0,
+                       double* g1);
+void or_philox_trial(const or_code* c, int bps, double noise_var, uint64_t seed, uint32_t point,
+                     uint64_t trial, uint8_t* info, double* yre, double* yim, double* llr);
 
 /* ---- statistics: crates/com-math/src/special.rs, crates/com-sim/src/stats.rs ---------- */
 double or_erfc(double x);                       /* special.rs:180-255 (Cody) */
@@ -122,6 +136,10 @@ void or_trial_bpsk_lifted(const or_code* c, double ebn0_db, uint64_t seed, const
    noise_var per symbol, max-log LLRs (E values). */
 void or_trial_nr_qam(const or_code* c, int m, double noise_var, uint64_t seed, const char* name,
                      uint64_t cond, uint64_t trial, uint8_t* info, double* llr);
+/* the same for count trials from trial0 on `threads` POSIX threads */
+void or_trial_nr_qam_many(const or_code* c, int m, double noise_var, uint64_t seed, const char* name,
+                          uint64_t cond, uint64_t trial0, int64_t count, uint8_t* info, double* llr,
+                          int threads);
 
 #ifdef __cplusplus
 }

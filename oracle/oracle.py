@@ -64,6 +64,7 @@ def _load():
         "or_decode_llr": (C.c_int, [C.c_void_p, C.POINTER(OrDec), _f64p, _f64p]),
         "or_nr_decode_block": (C.c_int, [C.c_void_p, C.POINTER(OrDec), _f64p, _u8p]),
         "or_decode_many": (None, [C.c_void_p, C.POINTER(OrDec), _f64p, C.c_int64, _f64p, _i32p, C.c_int]),
+        "or_trace_f32_f64": (C.c_int, [C.c_void_p, C.POINTER(OrDec), _f64p, _f64p]),
         "or_decode_llr_f32": (C.c_int, [C.c_void_p, C.POINTER(OrDec), _f32p, _f32p]),
         "or_decode_many_f32": (None, [C.c_void_p, C.POINTER(OrDec), _f32p, C.c_int64, _f32p, _i32p, C.c_int]),
         "or_f32_to_f16": (C.c_uint16, [C.c_float]),
@@ -80,6 +81,14 @@ def _load():
         "or_qam_awgn_llr": (None, [C.c_int, _u8p, C.c_int64, C.c_double, C.POINTER(OrRng), _f64p]),
         "or_philox4x32_10": (None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
         "or_philox_info_bits": (None, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_int32, _u8p]),
+        "or_philox_normals": (None, [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]),
+        "or_philox_trial": (None, [C.c_void_p, C.c_int, C.c_double, C.c_uint64, C.c_uint32, C.c_uint64, _u8p,
+                                   _f64p, _f64p, _f64p]),
+        "or_label_halves_llr": (C.c_double, [_f64p, C.c_int, C.c_int]),
+        "or_llr_from_metrics": (None, [_f64p, C.c_int, _f64p]),
+        "or_trial_nr_qam_many": (None, [C.c_void_p, C.c_int, C.c_double, C.c_uint64, C.c_char_p, C.c_uint64,
+                                        C.c_uint64, C.c_int64, _u8p, _f64p, C.c_int]),
         "or_erfc": (C.c_double, [C.c_double]),
         "or_erfcinv": (C.c_double, [C.c_double]),
         "or_qfuncinv": (C.c_double, [C.c_double]),
@@ -221,6 +230,19 @@ class Code:
         lib.or_decode_many_f32(self.h, C.byref(d), llr, len(llr), post, its, threads)
         return post, its
 
+    def trace_f32_f64(self, llr, rule=NMS, alpha=0.75, max_iterations=10):
+        """The layered min-sum in f64 and FP32 side by side on the same LLRs (values exact in
+        FP32): a dict of where the two runs part (or_trace_f32_f64)."""
+        d = OrDec(rule, alpha, 1, max_iterations)
+        out = np.zeros(10 + max_iterations)
+        if lib.or_trace_f32_f64(self.h, C.byref(d), np.ascontiguousarray(llr, np.float64), out):
+            raise ValueError("the side-by-side run takes the min-sum rules")
+        keys = ("it64", "it32", "min_abs", "first_sign_abs", "first_sign_it", "sign_events", "argmin_events",
+                "min_argmin_gap", "first_argmin_it", "decisions_differ")
+        r = {k: float(out[i]) for i, k in enumerate(keys)}
+        r["rel"] = out[10:].copy()
+        return r
+
     def decode_many_f16(self, llr, rule=NMS, alpha=0.75, max_iterations=20, threads=1):
         """The layered min-sum with the messages stored in binary16 and FP32 arithmetic.
         max_iterations: one limit, or one per codeword. Returns the posteriors (the stored
@@ -250,6 +272,22 @@ class Code:
         llr = np.zeros(self.e)
         lib.or_trial_nr_qam(self.h, m, noise_var, seed, name.encode(), cond, trial, info, llr)
         return info, llr
+
+    def trial_nr_qam_many(self, m, noise_var, seed, name, cond, trial0, count, threads=1):
+        """trial_nr_qam for trials trial0 .. trial0 + count - 1: (info (count, K'), llr (count, E))."""
+        info = np.zeros((count, self.k_info), np.uint8)
+        llr = np.zeros((count, self.e))
+        lib.or_trial_nr_qam_many(self.h, m, noise_var, seed, name.encode(), cond, trial0, count, info, llr, threads)
+        return info, llr
+
+    def philox_trial(self, bps, noise_var, seed, point, trial):
+        """One trial of the device trial generator restated in f64: (info, y, llr); y is
+        complex (BPSK: the real received values)."""
+        info = np.zeros(self.k_info, np.uint8)
+        nsym = self.e // bps
+        yre, yim, llr = np.zeros(nsym), np.zeros(nsym), np.zeros(self.e)
+        lib.or_philox_trial(self.h, bps, noise_var, seed, point, trial, info, yre, yim, llr)
+        return info, yre + 1j * yim, llr
 
     def learned_decode(self, llr, alpha, beta):
         alpha = np.ascontiguousarray(alpha, np.float64)
@@ -297,6 +335,43 @@ def philox_info_bits(seed, point, trial, k_info):
     bits = np.zeros(k_info, np.uint8)
     lib.or_philox_info_bits(seed, point, trial, k_info, bits)
     return bits
+
+
+def philox_normals(seed, point, trial, index):
+    g0, g1 = C.c_double(), C.c_double()
+    lib.or_philox_normals(seed, point, trial, index, C.byref(g0), C.byref(g1))
+    return g0.value, g1.value
+
+
+def label_halves_llr(metric, shift):
+    metric = np.ascontiguousarray(metric, np.float64)
+    return lib.or_label_halves_llr(metric, metric.size, shift)
+
+
+def llr_from_metrics(metric):
+    metric = np.ascontiguousarray(metric, np.float64)
+    out = np.zeros(int(np.log2(metric.size)))
+    lib.or_llr_from_metrics(metric, metric.size, out)
+    return out
+
+
+def qam_llr(m, y, noise_var):
+    y = np.asarray(y, np.complex128).ravel()
+    b = int(np.log2(m))
+    out = np.zeros(y.size * b)
+    lib.or_qam_llr(m, np.ascontiguousarray(y.real), np.ascontiguousarray(y.imag), y.size, noise_var, out)
+    return out
+
+
+def complex_gaussians(seed, count):
+    """count draws of Rng::complex_gaussian (rng.rs:161-167) from Rng::new(seed)."""
+    r = Rng(seed)
+    re, im = C.c_double(), C.c_double()
+    out = np.zeros(count, np.complex128)
+    for i in range(count):
+        lib.or_rng_complex_gaussian(C.byref(r.r), C.byref(re), C.byref(im))
+        out[i] = re.value + 1j * im.value
+    return out
 
 
 def qam_points(m):

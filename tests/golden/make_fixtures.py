@@ -12,6 +12,9 @@ outputs are committed, and nothing at test time reads the reference.
   of the short FECFRAME, rate 1/2 (ETSI EN 302 307-1 V1.4.1, Annex C), as the reference
   holds it (crates/com-systems/src/dvbs2/tables.rs, N16200_P9000): the code of the AFF3CT
   run ldpc_dvbs2_short_r1_2_layered_spa_i50.
+- gray_mapping_qam_{4,16,64}.json: the Gray-mapped square QAM points of OptiCommPy 0.11.0
+  (tests/oracle/gray_mapping_qam_*.json, used by tests/test_oracle.py:97-105): point i of
+  the unnormalised constellation is the symbol of label i.
 """
 import json
 import os
@@ -36,6 +39,9 @@ def main():
             shutil.copy(os.path.join(REF, "validation/aff3ct", name), os.path.join(OUT, "aff3ct_" + name))
     for name in ("peg_n576_k288.alist", "peg_n648_k540.alist"):
         shutil.copy(os.path.join(REF, "validation/aff3ct/matrices", name), os.path.join(OUT, name))
+    for m in (4, 16, 64):
+        name = f"gray_mapping_qam_{m}.json"
+        shutil.copy(os.path.join(REF, "tests/oracle", name), os.path.join(OUT, name))
     text = open(os.path.join(REF, "crates/com-systems/src/dvbs2/tables.rs")).read()
     body = re.search(r"const N16200_P9000: &\[u16\] = &\[(.*?)\];", text, re.S).group(1)
     table = [int(x) for x in body.split(",") if x.strip()]

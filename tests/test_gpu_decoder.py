@@ -2,9 +2,11 @@
 
 - f64 messages: bit-identical posteriors and iteration counts for the min-sum rules
   (the same IEEE operations in the same order, no FMA contraction).
-- f32 messages: identical hard decisions, frame and bit error counts and iteration
-  counts, posteriors within FP32 rounding of the f64 values (tolerances written below),
-  except on recorded ties (|posterior| below TIE).
+- f32 messages: identical iteration counts and frame-error verdicts on every frame,
+  identical hard decisions and bit-error counts on every frame without a recorded tie (a
+  value whose FP32 sign differs from its f64 sign, recorded by the oracle's side-by-side
+  run; TIE is the magnitude below which such a value is a tie on zero), posteriors within
+  the relative tolerances written in each test.
 """
 import numpy as np
 import pytest
@@ -208,30 +210,51 @@ def test_f32_bg1_z384_bit_exact_with_fp32_oracle(esn0_db):
     assert np.array_equal(bits.cpu().numpy(), (post_ref[:, :8448] < 0).astype(np.uint8))
 
 
-def test_f32_decisions_track_the_f64_reference():
+@pytest.mark.parametrize("esn0_db", [4.0, 4.5, 5.0, 6.0, 7.0])
+def test_f32_against_the_f64_reference_with_recorded_ties(esn0_db):
     """FP32 messages against the reference's f64 decoder on the same LLRs (BG1 Z=384,
-    16-QAM, 4.5 dB): the decisions, the frame errors and the iteration counts agree on
-    every frame whose f64 decoding converged without a near-tie (|v2c| tie frames are the
-    only ones where the FP32 rounding can flip a sign), and the posteriors of converged
-    frames agree to 1e-3 relative (measured spread of FP32 rounding over <= 10 iterations:
-    up to ~1.3e-3 on rare frames, typically 1e-5)."""
-    print("\n[f32 vs f64] BG1 Z=384 16-QAM 4.5 dB", flush=True)
-    count = 32
-    ref, infos, llrs = config2_llrs(4.5, count)
-    fulls = np.array([ref.full_llr(l.astype(np.float32).astype(np.float64)) for l in llrs])
-    post_ref, it_ref = ref.decode_many(fulls, rule=O.NMS, alpha=0.75, layered=True, max_iterations=10, threads=8)
+    16-QAM, layered NMS 0.75, 10 iterations; config 2's Es/N0 grid {4, 5, 6, 7} dB and the
+    4.5 dB point near FER 1e-3), 64 frames a point.
+
+    Ties are recorded by the oracle's side-by-side run of the two arithmetics
+    (or_trace_f32_f64): a frame has a recorded tie when a check-node input or a posterior
+    takes another sign in FP32 than in f64, the only way their decisions can part; its f64
+    magnitude is recorded too (below TIE on frames where one input sits on zero, up to
+    ~1e-3 where the FP32 rounding has grown over several iterations in the waterfall).
+    Every frame: the same iteration count and the same frame-error verdict. Frames without
+    a recorded tie: identical decisions and bit-error counts. Frames with one: decisions
+    may differ only on frames that fail in both. Posteriors: within 1e-4 relative (of
+    max(|f64|, 1)) on every frame at Es/N0 >= 5 dB (measured <= 4.9e-5), within 2e-3 on
+    converged frames below (measured <= 1.2e-3: the rounding difference grows about
+    five-fold an iteration over the first four, then saturates)."""
+    print(f"\n[f32 vs f64] BG1 Z=384 16-QAM {esn0_db} dB, recorded ties", flush=True)
+    count = 64
+    ref, infos, llrs = config2_llrs(esn0_db, count)
+    llr32 = llrs.astype(np.float32)
+    fulls = np.array([ref.full_llr(l.astype(np.float64)) for l in llr32])
+    post64, it64 = ref.decode_many(fulls, rule=O.NMS, alpha=0.75, layered=True, max_iterations=10, threads=8)
+    trace = [ref.trace_f32_f64(f, rule=O.NMS, alpha=0.75, max_iterations=10) for f in fulls]
+    tied = np.array([t["sign_events"] > 0 for t in trace])
+    below = sum(t["sign_events"] > 0 and t["first_sign_abs"] < TIE for t in trace)
     code = F.NrLdpc(8448, 25344, 1, 0, 4).set_decoder("normalized_min_sum", 0.75, True, 10)
-    bits, its, post = code.decode_batch(cuda(llrs.astype(np.float32)), want_post=True)
+    bits, its, post = code.decode_batch(cuda(llr32), want_post=True)
     bits, its, post = bits.cpu().numpy(), its.cpu().numpy(), post.cpu().numpy().astype(np.float64)
-    bits_ref = (post_ref[:, :8448] < 0).astype(np.uint8)
-    same_frames = (bits == bits_ref).all(1)
-    assert same_frames.mean() >= 0.95
-    assert np.array_equal((bits != infos).any(1)[same_frames], (bits_ref != infos).any(1)[same_frames])
-    conv = (it_ref < 10) & (its == it_ref)
-    assert conv.mean() >= 0.9
-    rel = np.abs(post - post_ref) / np.maximum(np.abs(post_ref), 1.0)
-    assert np.max(rel[conv]) < 1e-3
-    assert np.median(rel[conv]) < 1e-5
+    bits64 = (post64[:, :8448] < 0).astype(np.uint8)
+    fail32, fail64 = (bits != infos).any(1), (bits64 != infos).any(1)
+    print(f"  frames {count}, converged {int((it64 < 10).sum())}, recorded ties {int(tied.sum())} "
+          f"({below} first below {TIE:g})", flush=True)
+    assert np.array_equal(its, it64)
+    assert np.array_equal(its, np.array([int(t["it32"]) for t in trace]))
+    assert np.array_equal(fail32, fail64)
+    same = (bits == bits64).all(1)
+    assert same[~tied].all()
+    assert np.array_equal((bits != infos).sum(1)[~tied], (bits64 != infos).sum(1)[~tied])
+    assert (same | (fail32 & fail64)).all()
+    rel = np.abs(post - post64) / np.maximum(np.abs(post64), 1.0)
+    conv = it64 < 10
+    if esn0_db >= 5.0:
+        assert rel.max() < 1e-4, rel.max()
+    assert rel[conv].max() < 2e-3, rel[conv].max()
 
 
 @pytest.mark.parametrize("limit", [1, 2, 3])
@@ -253,3 +276,45 @@ def test_f32_posteriors_within_1e4_of_f64_over_the_first_iterations(limit):
     assert np.array_equal(its.cpu().numpy(), it64)
     assert np.array_equal(post < 0, post64 < 0)
     assert np.max(np.abs(post - post64) / np.maximum(np.abs(post64), 1.0)) < 1e-4
+
+
+def test_host_entry_points_from_eight_threads_on_one_handle():
+    """The runner calls decode_llr from rayon workers through &self (runner.rs:435-446): eight
+    host threads call cf_decode_llr, cf_decode_block and cf_app_into on one handle at once
+    (ctypes releases the interpreter lock during each call), every result bit-identical to
+    the oracle's f64 decoder."""
+    import threading
+    print("\n[threads] 8 host threads, one handle, f64 host entry points", flush=True)
+    ref, cws, full = config1_llrs(48, ebn0_db=1.25, name="threads")
+    post_ref, it_ref = ref.decode_many(full, rule=O.NMS, alpha=0.75, layered=True, max_iterations=20, threads=8)
+    code = F.NrLdpc.lifted(2, 52).set_decoder("normalized_min_sum", 0.75, True, 20)
+    results, errors = {}, []
+
+    def work(t):
+        try:
+            for i in range(t, 48, 8):
+                if i % 3 == 0:
+                    post, it = code.decode_llr(full[i])
+                    results[i] = ("llr", post, it)
+                elif i % 3 == 1:
+                    results[i] = ("block", code.decode_block(full[i, 104:]), None)
+                else:
+                    app, it = code.app(full[i, 104:])
+                    results[i] = ("app", app, it)
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(t,)) for t in range(8)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors
+    assert len(results) == 48
+    for i, (kind, value, it) in results.items():
+        if kind == "llr":
+            assert np.array_equal(value, post_ref[i]) and it == it_ref[i]
+        elif kind == "block":
+            assert np.array_equal(value, (post_ref[i, :520] < 0).astype(np.uint8))
+        else:
+            assert np.array_equal(value, post_ref[i, :520]) and it == it_ref[i]

@@ -99,3 +99,22 @@ def test_training_lowers_the_loss_and_beats_the_plain_rule():
     # the frames the fallback decoded are the plain decoder's own decisions
     pb = plain_bits.cpu().numpy()[failed]
     assert np.array_equal(bits_fb.cpu().numpy()[failed], pb)
+
+
+def test_training_again_in_the_same_process_is_finite_and_repeatable():
+    """Several trainings in one process with different iteration counts (8, 10, 7, then 8
+    again), the sequence that once left Adam's moments uninitialised: every loss finite, and
+    the second run at T = 8 repeats the first bit for bit (fixed seeds, fixed-order
+    reductions)."""
+    print("\n[learned] training T = 8, 10, 7, 8 in one process (BG1 Z=384 16-QAM)", flush=True)
+    code = F.NrLdpc(8448, 25344, 1, 0, 4).set_decoder("normalized_min_sum", 0.75, True, 10)
+    runs = []
+    for T in (8, 10, 7, 8):
+        params = F.initial_params(code, T)
+        hist = F.learned_train(code, params, steps=30, batch=256, lr=1e-2, modulation=F.QAM16, snr_kind=F.ESN0_DB,
+                               snr_lo=4.1, snr_hi=4.5, seed=11)
+        assert np.all(np.isfinite(hist)), (T, hist)
+        assert torch.isfinite(params).all()
+        runs.append((hist, params.cpu().numpy()))
+    assert np.array_equal(runs[0][0], runs[3][0])
+    assert np.array_equal(runs[0][1], runs[3][1])
