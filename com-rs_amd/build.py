@@ -16,7 +16,7 @@ LIB = os.path.join(OUT_DIR, "libcomfec_gpu.so")
 # the microbenchmarks of the HBM, LDS and VALU peaks (bench.py only; not part of the library)
 PEAKS_LIB = os.path.join(OUT_DIR, "libcomfec_peaks.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["nr_code.cpp", "capi.cpp", "sim.cpp", "sim_report.cpp", "campaign.cpp", "hooks_api.cpp", "learned_api.cpp", "batch.cpp", "decode_generic.hip", "decode_layered.hip", "decode_layered_f32.hip", "decode_layered_f16.hip", "trials.hip", "learned.hip"]
+SOURCES = ["nr_code.cpp", "capi.cpp", "sim.cpp", "sim_report.cpp", "campaign.cpp", "hooks_api.cpp", "learned_api.cpp", "batch.cpp", "decode_generic.hip", "decode_layered.hip", "decode_layered_f32.hip", "decode_layered_f16.hip", "trials.hip", "learned.hip", "static_scratch.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

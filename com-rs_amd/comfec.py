@@ -53,6 +53,8 @@ SIGNATURES = [
                               C.POINTER(C.c_uint32)]),
     ("cf_decode_batch", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p]),
+    ("cf_decode_batch_static", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p]),
 ]
 
 class StopNode(C.Structure):

@@ -124,6 +124,22 @@ cf_status cf_app_into(const cf_code* code, const double* llr_host, size_t e,
 cf_status cf_decode_batch(const cf_code* code, int msg_type, const void* d_llr, size_t count,
                           uint8_t* d_bits, void* d_post, uint32_t* d_iterations, void* stream);
 
+/* No counterpart in the reference: cf_decode_batch for callers that cannot allocate device
+   memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc). The
+   same arguments, kernels and results; every scratch buffer of the decode (the layered
+   kernel's task counter, the general kernels' check-to-variable messages) comes from one
+   array of CF_STATIC_SCRATCH_BYTES that the library holds on each GPU it runs on until the
+   process ends, and the call allocates and frees nothing. The code's tables must already be
+   on the device (any earlier decode or generator call with the code on that device puts them
+   there): otherwise CF_CONFIG. A code whose codeword needs more message scratch than the
+   array holds (the general kernel's f64 and FP32 paths, codes given by their rows) gets
+   CF_CONFIG; the general kernels run as many codewords side by side as the array holds.
+   Every call on a GPU shares that GPU's array: calls that overlap on one GPU (several
+   streams, or host threads) race and must be serialised by the caller. */
+#define CF_STATIC_SCRATCH_BYTES (64u << 20)
+cf_status cf_decode_batch_static(const cf_code* code, int msg_type, const void* d_llr, size_t count,
+                                 uint8_t* d_bits, void* d_post, uint32_t* d_iterations, void* stream);
+
 
 /* ---- batches of many codes: one persistent launch --------------------------------------- */
 
