@@ -24,12 +24,18 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "com-rs_amd"))
 
-HBM_PEAK_GBS = 8000.0
-# HBM bytes per codeword of the plain launch, from rocprofv3 FETCH_SIZE (x 2.0) and
-# WRITE_SIZE (x 1.0) on this workload (profiles/decode_layered_bg1_z384_16qam.md):
-# the channel LLRs in and the decisions out, 455.0 MB per 4096 codewords
+# MI355X published peaks: HBM3E 8 TB/s; LDS 128 B/clk per CU x 256 CUs x 2.4 GHz; VALU one
+# wave64 instruction per 4 clocks per SIMD x 1024 SIMDs x 2.4 GHz
+HBM_PUBLISHED_GBS = 8000.0
+LDS_PUBLISHED_GBS = 256 * 128 * 2.4
+VALU_PUBLISHED_GINST = 1024 * 2.4 / 4
+# What the profiles of the plain launch on this workload measured, per codeword
+# (profiles/decode_layered_bg1_z384_16qam.md): HBM bytes from rocprofv3 FETCH_SIZE (x 2.0)
+# and WRITE_SIZE (x 1.0), the channel LLRs in and the decisions out (455.0 MB per 4096
+# codewords); SQ_INSTS_VALU per codeword and iteration (6 waves)
 TRAFFIC_PER_CODEWORD = 111078
-LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz
+VALU_PER_CODEWORD_ITERATION = 6 * 5509
+PROFILE = "profiles/decode_layered_bg1_z384_16qam.md"
 LEARNED_ITERATIONS = 10
 K_INFO, E, Z, BG1_EDGES = 8448, 25344, 384, 316
 
@@ -46,7 +52,48 @@ def parse():
     return p.parse_args()
 
 
-def cpu_baseline(llr_f32, seconds):
+def measure_peaks(torch):
+    """The HBM, LDS and VALU peaks of this device, by the microbenchmarks of
+    com-rs_amd/csrc/peaks.hip (mean of 10 launches after a warm-up each)."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(ROOT, "com-rs_amd", "_build", "libcomfec_peaks.so"))
+    for name in ("cf_peak_hbm_read", "cf_peak_hbm_copy", "cf_peak_lds", "cf_peak_valu"):
+        getattr(lib, name).restype = C.c_double
+    sink = torch.zeros(1 << 16, dtype=torch.int32, device="cuda")
+    src = torch.ones(1 << 29, dtype=torch.int32, device="cuda")  # 2 GiB
+    dst = torch.empty(1 << 28, dtype=torch.int32, device="cuda")  # 1 GiB
+    w = C.c_double()
+    sp = C.c_void_p(sink.data_ptr())
+    out = {}
+    ms = lib.cf_peak_hbm_read(C.c_void_p(src.data_ptr()), C.c_size_t(src.numel() * 4), sp, 10, C.byref(w))
+    out["hbm_read_gbs"] = round(w.value / ms / 1e6, 1)
+    ms = lib.cf_peak_hbm_copy(C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()), C.c_size_t(dst.numel() * 4),
+                              10, C.byref(w))
+    out["hbm_copy_gbs"] = round(w.value / ms / 1e6, 1)
+    del src, dst
+    for width in (4, 16):
+        ms = lib.cf_peak_lds(width, 4096, sp, 10, C.byref(w))
+        out[f"lds_read_b{8 * width}_gbs"] = round(w.value / ms / 1e6, 1)
+    ms = lib.cf_peak_valu(20000, sp, 10, C.byref(w))
+    out["valu_ginst_s"] = round(w.value / ms / 1e6, 1)
+    torch.cuda.synchronize()
+    return out
+
+
+def host_threads():
+    """The host threads the CPU baseline may use: the CPUs this process may run on, at most
+    the box's share given by OMP_NUM_THREADS when that is set."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    cap = os.environ.get("OMP_NUM_THREADS")
+    if cap and cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(1, n)
+
+
+def cpu_baseline(llr_f32, seconds, esn0):
     """The oracle's f64 layered NMS (a C restatement of LdpcCode::propagate, 10 iterations,
     syndrome stop) on host threads over a bounded sample of the timed batch, on all cores
     and on one; and, once, the one published timing of the reference's decoder
@@ -57,7 +104,7 @@ def cpu_baseline(llr_f32, seconds):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     ref = O.Code.nr(K_INFO, E, 1, 0, 4)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_threads()
 
     def rate(count, nthreads):
         fulls = np.array([ref.full_llr(l.astype(np.float64)) for l in llr_f32[:count]])
@@ -91,7 +138,24 @@ def cpu_baseline(llr_f32, seconds):
         cal.decode_many(fulls, rule=O.SPA, alpha=0.0, layered=False, max_iterations=50, threads=1)
         runs.append(time.perf_counter() - t0)
     cal_s = float(np.median(runs))
+
+    # the whole trial chain on the same threads: xoshiro streams, bits, NR encoder, rate
+    # matching and interleaver, Gray 16-QAM, AWGN, max-log LLRs, rate recovery, the decoder,
+    # the bit and frame error counts
+    n0 = 1.0 / O.lib.or_db2lin(esn0)
+    chain = max(threads, count // 2)
+    t0 = time.perf_counter()
+    info, llr = ref.trial_nr_qam_many(16, n0, 1, "bench", 0, 0, chain, threads=threads)
+    fulls = np.array([ref.full_llr(l) for l in llr])
+    post, _ = ref.decode_many(fulls, rule=O.NMS, alpha=0.75, layered=True, max_iterations=10, threads=threads)
+    errs = ((post[:, :K_INFO] < 0).astype(np.uint8) != info).sum(1)
+    dt_chain = time.perf_counter() - t0
     return {"value": round(K_INFO * count / dt / 1e9, 6), "unit": "Gbit/s", "cores": threads, "kind": "port",
+            "host_cpus": os.cpu_count(),
+            "full_chain": {"value": round(K_INFO * chain / dt_chain / 1e9, 6), "unit": "Gbit/s",
+                           "sample": f"{chain} trials at Es/N0 {esn0} dB (bits, encoder, interleaver, 16-QAM, "
+                                     f"AWGN, max-log LLRs, decoder, error counts) on {threads} threads, "
+                                     f"{dt_chain:.1f} s, {int((errs > 0).sum())} frame errors"},
             "sample": f"{count} codewords of the timed batch: f64 layered NMS 0.75, 10 it, syndrome stop "
                       f"(oracle C restatement of LdpcCode::propagate), {threads} threads, {dt:.1f} s",
             "single_thread": {"value": round(K_INFO * one / dt1 / 1e9, 6), "unit": "Gbit/s",
@@ -188,13 +252,19 @@ def main():
     gbps_plain = K_INFO * batch / (ms_plain / 1e3) / 1e9
     gbps_learned = K_INFO * batch / (ms_learned / 1e3) / 1e9
 
-    # roofline of the layered kernel (one launch per plain step): algorithmic bytes per
-    # codeword = channel LLRs in + K/8 decisions out + per iteration every edge message
-    # read and written in FP32 (the HBM yardstick of a decoder that streams its messages)
+    # roofline of the layered kernel (one launch per plain step), against the resource that
+    # binds it: the posteriors live in LDS and the messages in registers, so its algorithmic
+    # bytes are LDS bytes, I x E_dg x 16 per codeword (posterior read and write, variable-to-
+    # check message write and read, 4 B each); the unit is one codeword, a launch decodes
+    # `batch` of them, at the mean iteration count I of the timed batch
+    peaks = measure_peaks(torch)
     edges = BG1_EDGES * Z
-    alg_bytes = batch * (E * 4 + math.ceil(K_INFO / 8) + it_plain * 2 * edges * 4)
-    achieved = alg_bytes / (ms_plain / 1e3) / 1e9
-    lds_bytes = batch * it_plain * edges * 8  # posterior read + write per edge and iteration
+    s_plain = ms_plain / 1e3
+    lds_bytes = batch * it_plain * edges * 16
+    lds_achieved = lds_bytes / s_plain / 1e9
+    lds_peak = peaks["lds_read_b32_gbs"]
+    valu_rate = batch * it_plain * VALU_PER_CODEWORD_ITERATION / s_plain / 1e9
+    hbm_yard = batch * (E * 4 + math.ceil(K_INFO / 8) + it_plain * 2 * edges * 4)
     wil = lambda fe, n: [round(x, 7) for x in F.wilson(fe, n, 0.95)]
     out = {
         "metric": "decoded Gbit/s at FER 1e-3, learned vs plain min-sum, NR BG1 Z=384",
@@ -232,20 +302,30 @@ def main():
             "plain_fer_by_esn0_grid": grid,
             "wall_s_timed": round(wall_plain + wall_learned, 4),
         },
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_PER_CODEWORD * batch,
+        "roofline": {"bound": "lds", "achieved": round(lds_achieved, 1), "peak": lds_peak, "unit": "GB/s",
+                     "frac": round(lds_achieved / lds_peak, 4), "traffic": TRAFFIC_PER_CODEWORD * batch,
                      "kernel": "decode_layered_kernel<1, false, true, 384> (plain step)",
-                     "note": "achieved counts every edge message read and written in HBM each iteration "
-                             "(the algorithmic yardstick); the kernel keeps the messages in registers and the "
-                             "posteriors in LDS, so frac above 1 is that saving; the HBM bytes it moves are "
-                             "`traffic` (LLRs in, decisions out)",
-                     "traffic_gbs": round(TRAFFIC_PER_CODEWORD * batch / (ms_plain / 1e3) / 1e9, 2),
-                     "lds": {"achieved": round(lds_bytes / (ms_plain / 1e3) / 1e9, 2), "peak": LDS_PEAK_GBS,
-                             "unit": "GB/s"}},
+                     "unit_of_work": f"one codeword, I x E_dg x 16 B of LDS (I = {it_plain:.3f}, E_dg = {edges}); "
+                                     f"{batch} a launch",
+                     "peak_source": "measured: conflict-free 4-byte LDS reads on every CU (peaks.hip)",
+                     "peaks_measured": peaks,
+                     "peaks_published": {"hbm_gbs": HBM_PUBLISHED_GBS, "lds_gbs": LDS_PUBLISHED_GBS,
+                                         "valu_ginst_s": VALU_PUBLISHED_GINST},
+                     "valu": {"ginst_s": round(valu_rate, 1), "peak": peaks["valu_ginst_s"],
+                              "frac": round(valu_rate / peaks["valu_ginst_s"], 4),
+                              "note": f"wave instructions per codeword and iteration from SQ_INSTS_VALU ({PROFILE}) "
+                                      "times the live iterations and batch over the live step time: the issue "
+                                      "rate that bounds the kernel"},
+                     "traffic_source": f"{PROFILE}: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (a 1 GiB copy's "
+                                       "calibration), the LLRs in and the decisions out",
+                     "hbm_yardstick": {"bytes": round(hbm_yard), "achieved_gbs": round(hbm_yard / s_plain / 1e9, 1),
+                                       "note": "the streaming decoder's HBM bytes (LLRs + K'/8 + I x 2 x E_dg x 4 per "
+                                               "codeword); this kernel keeps those messages on chip, so the figure "
+                                               "exceeds the HBM peak and is no roofline"}},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(llr.cpu().numpy(), args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(llr.cpu().numpy(), args.cpu_seconds, esn0)
     print(json.dumps(out), flush=True)
 
 

@@ -11,6 +11,7 @@ step measured with HIP events.
     python com-rs_amd/profile_decode.py --uneven 20000 --half --graph
     python com-rs_amd/profile_decode.py --copy-mib 1024 --steps 5
     python com-rs_amd/profile_decode.py --dvbs2 20000 --ebn0 0.9 --steps 2
+    python com-rs_amd/profile_decode.py --learned-sweep 6,7,8,9,10 --esn0 4.2
 
 Under rocprofv3, e.g.:
     rocprofv3 --kernel-trace --stats -d OUT -o plain -- python3 com-rs_amd/profile_decode.py
@@ -39,9 +40,17 @@ def main():
     p.add_argument("--dvbs2", type=int, default=0,
                    help="codewords of the DVB-S2 short rate-1/2 code (layered sum-product, FP32, 50 it)")
     p.add_argument("--ebn0", type=float, default=0.9, help="dvbs2: Eb/N0 in dB")
+    p.add_argument("--learned-sweep", default="",
+                   help="comma-separated iteration counts T: the learned decoder trained for each, then its FER and "
+                        "mean iterations (with and without the plain fallback) against the plain decoder and the "
+                        "sum-product rule on the same frames")
+    p.add_argument("--frames", type=int, default=32768, help="learned sweep: evaluation frames")
+    p.add_argument("--sweep-train-steps", type=int, default=1000, help="learned sweep: Adam steps for each T")
     a = p.parse_args()
     import torch
     import comfec as F
+    if a.learned_sweep:
+        return learned_sweep(a, torch, F)
     if a.uneven:
         return uneven(a, torch, F)
     if a.dvbs2:
@@ -82,6 +91,46 @@ def timed(fn, steps, torch):
     stop.record()
     torch.cuda.synchronize()
     return start.elapsed_time(stop) / steps
+
+
+def learned_sweep(a, torch, F):
+    """The learned decoder at several iteration counts T on the BG1 Z=384 16-QAM workload at
+    Es/N0 `esn0`: trained on the device as bench.py trains it (1000 Adam steps of 512
+    codewords, lr 1e-2, Es/N0 uniform in [esn0 - 0.1, esn0 + 0.3]), then decoded on `frames`
+    fresh frames with and without the plain fallback; the plain NMS 0.75 with 10 iterations,
+    and the layered NMS and FP32 sum-product with a limit of 50 on the same frames. One JSON
+    line per decoder: frames, frame errors, FER with its 95 % Wilson interval, mean
+    iterations, and the time of one 4096-codeword launch (HIP events)."""
+    import json
+    code = F.NrLdpc(8448, 25344, 1, 0, 4).set_decoder("normalized_min_sum", 0.75, True, 10)
+    nms50 = F.NrLdpc(8448, 25344, 1, 0, 4).set_decoder("normalized_min_sum", 0.75, True, 50)
+    spa50 = F.NrLdpc(8448, 25344, 1, 0, 4).set_decoder("sum_product", 0.0, True, 50)
+    parts = [F.generate_batch(code, 4096, F.QAM16, a.esn0, snr_kind=F.ESN0_DB, seed=6, point=0, trial0=4096 * i)
+             for i in range(a.frames // 4096)]
+
+    def evaluate(name, fn):
+        fe, its_sum = 0, 0.0
+        for llr, info in parts:
+            bits, its = fn(llr)
+            fe += int((bits.cpu().numpy() != F.unpack_info(info, 8448)).any(1).sum())
+            its_sum += float(its.double().sum().item())
+        n = len(parts) * 4096
+        ms = timed(lambda: fn(parts[0][0]), 5, torch)
+        row = {"decoder": name, "esn0_db": a.esn0, "frames": n, "frame_errors": fe, "fer": fe / n,
+               "fer_wilson95": [round(x, 8) for x in F.wilson(fe, n, 0.95)], "mean_iterations": round(its_sum / n, 4),
+               "ms_per_4096": round(ms, 4), "gbit_s": round(8448 * 4096 / ms / 1e6, 3)}
+        print(json.dumps(row), flush=True)
+        return row
+
+    evaluate("plain NMS 0.75, 10 it", lambda l: code.decode_batch(l)[:2])
+    evaluate("NMS 0.75, limit 50", lambda l: nms50.decode_batch(l)[:2])
+    evaluate("sum-product FP32, limit 50", lambda l: spa50.decode_batch(l)[:2])
+    for T in [int(t) for t in a.learned_sweep.split(",")]:
+        params = F.initial_params(code, T)
+        F.learned_train(code, params, steps=a.sweep_train_steps, batch=512, lr=1e-2,
+                        modulation=F.QAM16, snr_kind=F.ESN0_DB, snr_lo=a.esn0 - 0.1, snr_hi=a.esn0 + 0.3, seed=11)
+        evaluate(f"learned T={T}", lambda l: F.learned_decode_batch(code, params, l, fallback=False)[:2])
+        evaluate(f"learned T={T} + plain fallback", lambda l: F.learned_decode_batch(code, params, l, fallback=True)[:2])
 
 
 LIFTS = [16, 18, 20, 22, 24, 26, 28, 30, 32, 36, 40, 44, 48, 52, 56, 60, 64, 72, 80, 88, 96, 104, 112, 120, 128,
