@@ -64,8 +64,9 @@ void run_layered(const cf_code* h, int device, const LayeredRun& r, hipStream_t 
     if (r.count <= 0) return;
     const int threads = std::min(round64(int(c.z)), kMaxThreads);
     const int g = group_of(int(c.z), threads);
+    const DecoderSettings st = h->settings();
     LayeredArgs a{};
-    a.one = batch_class(c, h->settings(), device, g);
+    a.one = batch_class(c, st, device, g);
     a.n_tasks = int32_t((r.count + g - 1) / g);
     a.count = r.count;
     a.params = r.params;
@@ -83,12 +84,14 @@ void run_layered(const cf_code* h, int device, const LayeredRun& r, hipStream_t 
     // workgroup for the next
     const bool solo = g == 1;
     const int zc = c.z == 384 && solo ? 384 : 0;
+    // the normalized min-sum's own instantiation (no offset, no learned parameters)
+    const bool nms = zc == 384 && !r.params && st.rule == 1;
     const size_t lds = off + kSharedBytes;
     int per_cu = 0, cus = 0;
-    check_hip(layered_occupancy(c.bg, r.half, solo, zc, threads, lds, &per_cu), "occupancy");
+    check_hip(layered_occupancy(c.bg, r.half, solo, zc, threads, lds, &per_cu, nms), "occupancy");
     check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "device attribute");
     const int grid = int(std::min<int64_t>(a.n_tasks, int64_t(std::max(per_cu, 1)) * std::max(cus, 1)));
-    LayeredLaunch l{c.bg, r.half, solo, zc, threads, std::max(grid, 1), lds};
+    LayeredLaunch l{c.bg, r.half, solo, zc, threads, std::max(grid, 1), lds, nms};
     if (counter) {
         // the caller's counter: a fill kernel (no memset node) zeroes it before the launch
         check_hip(launch_zero(reinterpret_cast<uint64_t*>(counter), 1, stream), "zero launch");

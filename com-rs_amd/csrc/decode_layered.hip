@@ -6,14 +6,14 @@
 
 namespace comfec {
 
-const void* layered_kernel_f32(int bg_mask, bool solo, int zc);
-const void* layered_kernel_f16(int bg_mask, bool solo, int zc);
+const void* layered_kernel_f32(int bg_mask, bool solo, int zc, bool nms);
+const void* layered_kernel_f16(int bg_mask, bool solo, int zc, bool nms);
 
 namespace {
 
-const void* select_kernel(int bg_mask, bool half, bool solo, int zc) {
+const void* select_kernel(int bg_mask, bool half, bool solo, int zc, bool nms) {
     if (bg_mask < 1 || bg_mask > 3) return nullptr;
-    return half ? layered_kernel_f16(bg_mask, solo, zc) : layered_kernel_f32(bg_mask, solo, zc);
+    return half ? layered_kernel_f16(bg_mask, solo, zc, nms) : layered_kernel_f32(bg_mask, solo, zc, nms);
 }
 
 }  // namespace
@@ -29,8 +29,8 @@ size_t layered_lds_bytes(int bg, int z, int g, bool half) {
 }
 
 hipError_t layered_occupancy(int bg_mask, bool half, bool solo, int zc, int threads, size_t lds,
-                             int* blocks_per_cu) {
-    const void* k = select_kernel(bg_mask, half, solo, zc);
+                             int* blocks_per_cu, bool nms) {
+    const void* k = select_kernel(bg_mask, half, solo, zc, nms);
     if (!k) return hipErrorInvalidValue;
     hipError_t err = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (err != hipSuccess) return err;
@@ -38,7 +38,7 @@ hipError_t layered_occupancy(int bg_mask, bool half, bool solo, int zc, int thre
 }
 
 hipError_t layered_prepare(const LayeredLaunch& l) {
-    const void* k = select_kernel(l.bg_mask, l.half, l.solo, l.zc);
+    const void* k = select_kernel(l.bg_mask, l.half, l.solo, l.zc, l.nms);
     if (!k || l.lds > kLdsBudget) return hipErrorInvalidValue;
     if (l.zc) {
         // these kernels address the posteriors from LDS address 0: no static LDS before them
@@ -51,7 +51,7 @@ hipError_t layered_prepare(const LayeredLaunch& l) {
 }
 
 hipError_t launch_layered(const LayeredArgs& a, const LayeredLaunch& l, hipStream_t stream, bool prepare) {
-    const void* k = select_kernel(l.bg_mask, l.half, l.solo, l.zc);
+    const void* k = select_kernel(l.bg_mask, l.half, l.solo, l.zc, l.nms);
     if (!k || l.threads < 64 || l.threads > kMaxThreads || l.threads % 64 != 0 || l.grid < 1)
         return hipErrorInvalidValue;
     if (l.lds > kLdsBudget) return hipErrorInvalidValue;

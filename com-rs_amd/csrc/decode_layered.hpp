@@ -244,7 +244,7 @@ struct Messages<ROWS, true> {
 // The stored state keeps the output sign bits (edge q at bit deg-1-q, shifted in with a
 // funnel shift) and one index whose magnitude is g(m2); a message is rebuilt as that
 // magnitude with its sign bit replaced by the stored one, which is the output bit for bit.
-template <int BG, bool HALF, bool SOLO, int ZC, int I>
+template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, int I>
 __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post, int r, uint32_t gofs, int z,
                                       const uint32_t* __restrict__ shift, Messages<Graph<BG>::rows, HALF>& msg,
                                       float a, float b) {
@@ -316,10 +316,10 @@ __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post
             __builtin_amdgcn_bitop3_b32(pko << (32 - deg + q), __float_as_uint(mag), 0x80000000u, 0xe4));
         v2c[q] = load(ad) - old;
     }
-    // the row's sign product: three-input XORs
-    uint32_t parity = 0;
+    // the row's sign product: three-input XORs from the first input on
+    uint32_t parity = __float_as_uint(v2c[0]);
 #pragma unroll
-    for (int q = 0; q < deg; q += 2)
+    for (int q = 1; q < deg; q += 2)
         parity = q + 1 < deg ? __builtin_amdgcn_bitop3_b32(parity, __float_as_uint(v2c[q]),
                                                            __float_as_uint(v2c[q + 1]), 0x96)
                              : parity ^ __float_as_uint(v2c[q]);
@@ -344,8 +344,9 @@ __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post
         m2 = __builtin_amdgcn_fmed3f(m1, m2, m);
         m1 = fminf(m1, m);
     }
-    float g1 = a * fmaxf(m1 - b, 0.0f);
-    float g2 = a * fmaxf(m2 - b, 0.0f);
+    // NMS: b = 0, and a max(m - 0, 0) is a m bit for bit for every magnitude m >= +0
+    float g1 = NMS ? a * m1 : a * fmaxf(m1 - b, 0.0f);
+    float g2 = NMS ? a * m2 : a * fmaxf(m2 - b, 0.0f);
     msg.template put<I>(g1, g2);
     // the magnitudes with the row's sign product applied once: an edge's output is then
     // its choice XOR its own input's sign bit, and the stored bit is the output's sign
@@ -370,20 +371,20 @@ __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post
     msg.pk[I] = signs | (i1 << 24);
 }
 
-template <int BG, bool HALF, bool SOLO, int ZC, int I>
+template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, int I>
 __device__ __forceinline__ void sweep(typename Store<HALF>::T* post, int r, uint32_t gofs, int z, bool active,
                                       const uint32_t* shift, Messages<Graph<BG>::rows, HALF>& msg, const float* ab,
                                       float a, float b) {
     if constexpr (I < Graph<BG>::rows) {
-        if (ab) {
+        if (!NMS && ab) {
             a = ab[2 * I];
             b = ab[2 * I + 1];
         }
         if constexpr (Sched<BG>::s.barrier[I]) __syncthreads();
         // (the branch also bounds the scheduler's reach: without it the compiler interleaves
         // layers and spills hundreds of registers)
-        if (active) layer<BG, HALF, SOLO, ZC, I>(post, r, gofs, z, shift, msg, a, b);
-        sweep<BG, HALF, SOLO, ZC, I + 1>(post, r, gofs, z, active, shift, msg, ab, a, b);
+        if (active) layer<BG, HALF, SOLO, ZC, NMS, I>(post, r, gofs, z, shift, msg, a, b);
+        sweep<BG, HALF, SOLO, ZC, NMS, I + 1>(post, r, gofs, z, active, shift, msg, ab, a, b);
     }
 }
 
@@ -710,7 +711,7 @@ __device__ __forceinline__ void place_direct(typename Store<HALF>::T* post, cons
 // One task: codewords order[first .. first + cnt) of class c (base graph BG). SOLO: the
 // task has one codeword (G = 1), whose posteriors start the LDS; ZC: Z, when every class of
 // the launch has the same (0: read at run time).
-template <int BG, bool HALF, bool SOLO, int ZC>
+template <int BG, bool HALF, bool SOLO, int ZC, bool NMS>
 __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass& c, uint32_t first, int cnt, Shared& sh,
                          void* lds_dyn, uint32_t& stamp, uint32_t& round) {
     using S = Store<HALF>;
@@ -755,7 +756,9 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
         finished = true;  // the channel LLRs are a codeword: 0 iterations
         ok = 1;
     }
-    bool learned = a.params != nullptr;
+    // NMS instantiations serve launches without learned parameters whose classes are all
+    // normalized min-sum (b = 0)
+    bool learned = !NMS && a.params != nullptr;
     int it = 0, offset = 0;
     for (;;) {
         if (learned && it >= a.learned_iterations) {
@@ -781,7 +784,7 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
         if (!any_lane(sh, !finished, round)) break;
         ++it;
         const float* ab = learned ? a.params + size_t(it - 1) * 2 * Graph<BG>::rows : nullptr;
-        sweep<BG, HALF, SOLO, ZC, 0>(post_all, r, uint32_t(g * n), z, !finished, sh.shift, msg, ab, c.a, c.b);
+        sweep<BG, HALF, SOLO, ZC, NMS, 0>(post_all, r, uint32_t(g * n), z, !finished, sh.shift, msg, ab, c.a, c.b);
         __syncthreads();
         const bool fails = fails_now();
         if (!finished && !fails) {
@@ -840,8 +843,9 @@ __device__ __forceinline__ BatchClass uni(const BatchClass& c) {
 }
 
 // Workgroups take tasks until none is left. BGMASK: the base graphs of the classes; SOLO:
-// every class has G = 1; ZC: every class has Z = ZC (0: any).
-template <int BGMASK, bool HALF, bool SOLO, int ZC>
+// every class has G = 1; ZC: every class has Z = ZC (0: any); NMS: every class runs the
+// normalized min-sum and no learned parameters are given.
+template <int BGMASK, bool HALF, bool SOLO, int ZC, bool NMS = false>
 __global__ void __launch_bounds__(kMaxThreads, HALF ? 3 : 1) decode_layered_kernel(LayeredArgs a) {
     // one dynamic allocation and no static LDS, so that the posteriors start at address 0
     // and the column bases are immediate offsets of the LDS instructions
@@ -904,10 +908,10 @@ __global__ void __launch_bounds__(kMaxThreads, HALF ? 3 : 1) decode_layered_kern
             loaded = cls;
         }
         if constexpr ((BGMASK & 1) != 0) {
-            if (c.bg == 1) run_task<1, HALF, SOLO, ZC>(a, c, first, cnt, sh, lds_dyn, stamp, round);
+            if (c.bg == 1) run_task<1, HALF, SOLO, ZC, NMS>(a, c, first, cnt, sh, lds_dyn, stamp, round);
         }
         if constexpr ((BGMASK & 2) != 0) {
-            if (c.bg == 2) run_task<2, HALF, SOLO, ZC>(a, c, first, cnt, sh, lds_dyn, stamp, round);
+            if (c.bg == 2) run_task<2, HALF, SOLO, ZC, NMS>(a, c, first, cnt, sh, lds_dyn, stamp, round);
         }
         if (!a.queue) t += gridDim.x;
     }

@@ -4,9 +4,13 @@
 
 namespace comfec {
 
-const void* layered_kernel_f32(int bg_mask, bool solo, int zc) {
+const void* layered_kernel_f32(int bg_mask, bool solo, int zc, bool nms) {
     using namespace layered;
     if (solo && zc == 384) {
+        if (nms) {
+            if (bg_mask == 1) return reinterpret_cast<const void*>(decode_layered_kernel<1, false, true, 384, true>);
+            if (bg_mask == 2) return reinterpret_cast<const void*>(decode_layered_kernel<2, false, true, 384, true>);
+        }
         if (bg_mask == 1) return reinterpret_cast<const void*>(decode_layered_kernel<1, false, true, 384>);
         if (bg_mask == 2) return reinterpret_cast<const void*>(decode_layered_kernel<2, false, true, 384>);
     }
