@@ -74,8 +74,9 @@ def measure_peaks(torch):
     for width in (4, 16):
         ms = lib.cf_peak_lds(width, 4096, sp, 10, C.byref(w))
         out[f"lds_read_b{8 * width}_gbs"] = round(w.value / ms / 1e6, 1)
-    ms = lib.cf_peak_valu(20000, sp, 10, C.byref(w))
-    out["valu_ginst_s"] = round(w.value / ms / 1e6, 1)
+    for kind, name in ((0, "valu_ginst_s"), (1, "valu_int_add_ginst_s"), (2, "valu_bitop3_ginst_s")):
+        ms = lib.cf_peak_valu(kind, 20000, sp, 10, C.byref(w))
+        out[name] = round(w.value / ms / 1e6, 1)
     torch.cuda.synchronize()
     return out
 

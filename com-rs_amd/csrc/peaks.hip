@@ -3,7 +3,8 @@
 //   - HBM: a read-only stream (16-byte loads, a fold of what is read) and a copy;
 //   - LDS: 4-byte and 16-byte reads at consecutive addresses (no bank conflicts) by every
 //     wave a CU holds, the read width of the decoder's posterior accesses and the widest;
-//   - VALU issue: independent chains of FP32 fused multiply-adds on every SIMD.
+//   - VALU issue: independent chains of FP32 fused multiply-adds, 32-bit integer adds and
+//     three-input bitwise operations on every SIMD.
 // Each entry point times `launches` launches with HIP events after one warm-up launch and
 // returns the mean time of a launch in milliseconds, and the work of one launch (bytes or
 // wave instructions) through *work.
@@ -91,6 +92,26 @@ __global__ void __launch_bounds__(kThreads) valu_kernel(int iterations, uint32_t
     if (acc == -1.0f) sink[blockIdx.x] = 1u;
 }
 
+// Eight independent chains of a 32-bit integer instruction per lane: v_add_u32 (kind 0) or a
+// three-input v_bitop3_b32 (kind 1); the operand is made opaque every iteration, so the chain
+// cannot be folded.
+template <int KIND>
+__global__ void __launch_bounds__(kThreads) valu_int_kernel(int iterations, uint32_t* __restrict__ sink) {
+    uint32_t r[8], inc = threadIdx.x | 1u;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) r[k] = threadIdx.x * 17u + uint32_t(k);
+    for (int it = 0; it < iterations; ++it) {
+        asm volatile("" : "+v"(inc));
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            r[k] = KIND == 0 ? r[k] + inc : __builtin_amdgcn_bitop3_b32(r[k], inc, 0x5bd1e995u, 0x96);
+    }
+    uint32_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc ^= r[k];
+    if (acc == 0x9E3779B9u) sink[blockIdx.x] = acc;
+}
+
 template <typename L>
 double timed(int launches, L&& launch) {
     hipEvent_t a, b;
@@ -154,10 +175,19 @@ double cf_peak_lds(int width, int iterations, uint32_t* d_sink, int launches, do
     });
 }
 
-// VALU issue; *work = the wave instructions of the chains a launch.
-double cf_peak_valu(int iterations, uint32_t* d_sink, int launches, double* work) {
+// VALU issue; *work = the wave instructions of the chains a launch. kind 0: FP32 fused
+// multiply-add; 1: 32-bit integer add; 2: three-input bitwise operation.
+double cf_peak_valu(int kind, int iterations, uint32_t* d_sink, int launches, double* work) {
     const unsigned grid = unsigned(cus() * 8 * 4);
     *work = double(grid) * (kThreads / 64) * double(iterations) * 8.0;
+    if (kind == 1)
+        return timed(launches, [&] {
+            hipLaunchKernelGGL(valu_int_kernel<0>, dim3(grid), dim3(kThreads), 0, nullptr, iterations, d_sink);
+        });
+    if (kind == 2)
+        return timed(launches, [&] {
+            hipLaunchKernelGGL(valu_int_kernel<1>, dim3(grid), dim3(kThreads), 0, nullptr, iterations, d_sink);
+        });
     return timed(launches, [&] {
         hipLaunchKernelGGL(valu_kernel, dim3(grid), dim3(kThreads), 0, nullptr, iterations, d_sink);
     });
