@@ -827,8 +827,33 @@ def learned_decode_batch(code, params, llr, fallback=True, stream=None):
 class PlanDims(C.Structure):
     _fields_ = [("llr_total", C.c_uint64), ("bits_total", C.c_uint64), ("codewords", C.c_uint32),
                 ("classes", C.c_uint32), ("tasks", C.c_uint32), ("threads", C.c_uint32), ("grid", C.c_uint32),
-                ("lds_bytes", C.c_uint32), ("msg_type", C.c_int)]
+                ("lds_bytes", C.c_uint32), ("msg_type", C.c_int), ("segments", C.c_uint32)]
 
+
+class PlanConfig(C.Structure):
+    """cf_batch_plan_config: one launch at `threads` lanes, or (per_code_width) each code at
+    its own width; grid_per_cu 0 = as many resident workgroups as fit."""
+    _fields_ = [("threads", C.c_uint32), ("per_code_width", C.c_uint32), ("grid_per_cu", C.c_uint32),
+                ("msg_type", C.c_int), ("overlap", C.c_uint32)]
+
+    def __repr__(self):
+        layout = "per-code widths" if self.per_code_width else f"{self.threads} lanes"
+        if self.per_code_width:
+            layout += ", overlapped" if self.overlap else ", in turn"
+        grid = f"{self.grid_per_cu}/CU" if self.grid_per_cu else "all resident"
+        return f"PlanConfig({layout}, grid {grid}, {'FP16' if self.msg_type == MSG_F16 else 'FP32'})"
+
+
+class PlanSegment(C.Structure):
+    _fields_ = [("threads", C.c_uint32), ("grid", C.c_uint32), ("lds_bytes", C.c_uint32), ("classes", C.c_uint32),
+                ("tasks", C.c_uint32), ("codewords", C.c_uint32)]
+
+
+class PlanTrial(C.Structure):
+    _fields_ = [("config", PlanConfig), ("segments", C.c_uint32), ("ms", C.c_double)]
+
+
+TUNE_STORAGE = 1
 
 SIGNATURES += [
     ("cf_batch_plan_create", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
@@ -836,7 +861,11 @@ SIGNATURES += [
     ("cf_batch_plan_destroy", None, [C.c_void_p]),
     ("cf_batch_plan_dims_get", C.c_int, [C.c_void_p, C.POINTER(PlanDims)]),
     ("cf_batch_plan_set_threads", C.c_int, [C.c_void_p, C.c_uint32]),
+    ("cf_batch_plan_config_get", C.c_int, [C.c_void_p, C.POINTER(PlanConfig)]),
+    ("cf_batch_plan_configure", C.c_int, [C.c_void_p, C.POINTER(PlanConfig)]),
+    ("cf_batch_plan_segment_get", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(PlanSegment)]),
     ("cf_batch_plan_tune", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                     C.c_uint32, C.POINTER(PlanTrial), C.c_size_t, C.POINTER(C.c_size_t),
                                      C.POINTER(C.c_double), C.c_void_p]),
     ("cf_batch_plan_run", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                     C.c_void_p]),
@@ -889,12 +918,49 @@ class BatchPlan:
         _check(lib().cf_batch_plan_set_threads(self._h, threads))
         return self
 
-    def tune(self, llr, out=None, repeats=3, stream=None):
-        """Autotune the workgroup width on this input; returns (best ms, outputs)."""
+    @property
+    def config(self):
+        c = PlanConfig()
+        _check(lib().cf_batch_plan_config_get(self._h, C.byref(c)))
+        return c
+
+    def configure(self, threads=None, per_code_width=None, grid_per_cu=None, half=None, overlap=None):
+        """Change the configuration; the settings not given are kept."""
+        c = self.config
+        if overlap is not None:
+            c.overlap = int(overlap)
+        if threads is not None:
+            c.threads = threads
+        if per_code_width is not None:
+            c.per_code_width = int(per_code_width)
+        if grid_per_cu is not None:
+            c.grid_per_cu = grid_per_cu
+        if half is not None:
+            c.msg_type = MSG_F16 if half else MSG_F32
+        _check(lib().cf_batch_plan_configure(self._h, C.byref(c)))
+        return self
+
+    def segments(self):
+        """The launches of a run (PlanSegment each)."""
+        out = []
+        for i in range(self.dims.segments):
+            sg = PlanSegment()
+            _check(lib().cf_batch_plan_segment_get(self._h, i, C.byref(sg)))
+            out.append(sg)
+        return out
+
+    def tune(self, llr, out=None, repeats=3, storage=False, stream=None):
+        """Autotune on this input: returns (best ms, outputs); the trials (PlanTrial each) are
+        left in self.trials. storage: also try the other storage type."""
         bits, its, ok = out if out is not None else self.outputs()
         ms = C.c_double()
-        _check(lib().cf_batch_plan_tune(self._h, _ptr(llr), _ptr(bits), _ptr(its), _ptr(ok), repeats, C.byref(ms),
+        cap = 64
+        trials = (PlanTrial * cap)()
+        n = C.c_size_t()
+        _check(lib().cf_batch_plan_tune(self._h, _ptr(llr), _ptr(bits), _ptr(its), _ptr(ok), repeats,
+                                        TUNE_STORAGE if storage else 0, trials, cap, C.byref(n), C.byref(ms),
                                         _stream_ptr(stream)))
+        self.trials = [trials[i] for i in range(min(n.value, cap))]
         return ms.value, (bits, its, ok)
 
     def run(self, llr, out=None, graph=False, stream=None):

@@ -114,10 +114,18 @@ void run_layered(const cf_code* h, int device, const LayeredRun& r, hipStream_t 
 
 // ---- plans for batches of many codes ---------------------------------------------------
 
+// One launch of a plan's run: the tasks of the classes at one width.
+struct PlanSegment {
+    int threads = 0;
+    uint32_t classes = 0, codewords = 0;
+    size_t first_task = 0, n_tasks = 0;
+    LayeredLaunch launch{};
+    LayeredArgs args{};
+};
+
 struct cf_batch_plan {
     std::mutex mu;
     int device = 0;
-    bool half = false;
     size_t count = 0;
     uint64_t llr_total = 0, bits_total = 0;
     std::vector<const cf_code*> class_code;  // per class
@@ -126,16 +134,22 @@ struct cf_batch_plan {
     std::vector<uint16_t> max_it;
     std::vector<uint64_t> llr_off, bit_off;
     int bg_mask = 0, max_z = 0;
-    // the configuration
-    int threads = 0;
+    // the configuration as set, and what it resolved to
+    cf_batch_plan_config cfg{};
+    bool half = false;
+    std::vector<int> class_width;  // per class
     std::vector<BatchClass> classes;
-    std::vector<BatchTask> tasks;
+    std::vector<BatchTask> tasks;  // segment by segment
     std::vector<uint32_t> order;
-    LayeredLaunch launch{};
-    // device copies: classes, tasks, order, offsets, limits and the task counter
+    std::vector<PlanSegment> segs;
+    // device copies: task counters, classes, tasks, order, offsets and limits
     void* dev = nullptr;
     size_t dev_bytes = 0;
-    LayeredArgs args{};
+    uint64_t* counters = nullptr;
+    // overlap: a stream per launch after the first, the fork event and a join event each
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> joins;
+    hipEvent_t fork = nullptr;
     // the captured graph and the buffers it was captured with
     hipStream_t capture_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -148,6 +162,9 @@ struct cf_batch_plan {
     ~cf_batch_plan() {
         drop_graph();
         if (capture_stream) (void)hipStreamDestroy(capture_stream);
+        for (hipStream_t q : side) (void)hipStreamDestroy(q);
+        for (hipEvent_t e : joins) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
         if (dev) (void)hipFree(dev);
     }
     void drop_graph() {
@@ -160,71 +177,144 @@ struct cf_batch_plan {
 
 namespace {
 
+constexpr size_t kMaxSegments = 256 / sizeof(uint64_t);  // the counters fill the buffer's first 256 bytes
+static_assert(kMaxThreads / 64 <= int(kMaxSegments), "a counter per width");
+
 size_t up256(size_t x) { return (x + 255) & ~size_t(255); }
 
-// Tasks for a workgroup width: every code's codewords in groups of G, those with the larger
-// iteration limits together, and the tasks in decreasing order of work (the largest first,
-// so that the tail of the launch is short).
-void configure(cf_batch_plan& p, int threads) {
-    if (threads < round64(p.max_z) || threads > kMaxThreads || threads % 64 != 0)
+int multiprocessors(int device) {
+    int cus = 0;
+    check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "device attribute");
+    return std::max(cus, 1);
+}
+
+// The width for one code under per_code_width: of the multiples of 64 from Z (rounded up)
+// to 384, the one with the most codeword rows per busy SIMD, g Z x (resident workgroups) /
+// max(ceil(waves on the CU / 4), 2), at the occupancy the device reports for that width's
+// LDS. A SIMD with one wave hides little of the LDS latency, so it counts as no faster per
+// wave than one with two; ties go to the wider width, which has fewer tasks.
+int modelled_width(const NrCode& c, bool half, uint32_t cap_per_cu) {
+    int best_w = 0;
+    double best = -1.0;
+    for (int w = kMaxThreads; w >= round64(int(c.z)); w -= 64) {
+        const int g = group_of(int(c.z), w);
+        const size_t lds = layered_shared_offset(layered_lds_bytes(c.bg, int(c.z), g, half)) + kSharedBytes;
+        int per_cu = 0;
+        if (layered_occupancy(c.bg, half, g == 1, 0, w, lds, &per_cu) != hipSuccess || per_cu < 1) {
+            (void)hipGetLastError();
+            continue;
+        }
+        if (cap_per_cu) per_cu = std::min(per_cu, int(cap_per_cu));
+        const int busy = (per_cu * (w / 64) + 3) / 4;
+        const double rate = double(per_cu) * double(g) * double(c.z) / double(std::max(busy, 2));
+        if (rate > best * (1.0 + 1e-12)) {
+            best = rate;
+            best_w = w;
+        }
+    }
+    if (!best_w) fail(Kind::Device, "no workgroup width fits the device for Z = " + std::to_string(c.z));
+    return best_w;
+}
+
+void check_config(const cf_batch_plan& p, const cf_batch_plan_config& cfg) {
+    if (cfg.msg_type != CF_MSG_F32 && cfg.msg_type != CF_MSG_F16)
+        fail(Kind::Config, "a batch plan decodes with CF_MSG_F32 or CF_MSG_F16 messages, got " +
+                               std::to_string(cfg.msg_type));
+    if (cfg.per_code_width > 1) fail(Kind::Config, "per_code_width is 0 or 1");
+    if (cfg.overlap > 1) fail(Kind::Config, "overlap is 0 or 1");
+    const int t = int(cfg.threads);
+    if (!cfg.per_code_width && (t < round64(p.max_z) || t > kMaxThreads || t % 64 != 0))
         fail(Kind::Config, "the workgroup width must be a multiple of 64 from " + std::to_string(round64(p.max_z)) +
-                               " to " + std::to_string(kMaxThreads) + ", got " + std::to_string(threads));
+                               " to " + std::to_string(kMaxThreads) + ", got " + std::to_string(t));
+}
+
+// Tasks for a configuration: each code's codewords in groups of G = floor(width / Z), those
+// with the larger iteration limits together; per width (the widest first) one launch whose
+// tasks run in decreasing order of work (the largest first, so that its tail is short).
+void configure(cf_batch_plan& p, const cf_batch_plan_config& cfg) {
+    check_config(p, cfg);
     const size_t nc = p.class_code.size();
-    p.classes.assign(nc, BatchClass{});
-    size_t lds = 0;
-    bool solo = true, all384 = true;
+    const bool half = cfg.msg_type == CF_MSG_F16;
+    const int cus = multiprocessors(p.device);
+    std::vector<int> width(nc);
+    for (size_t k = 0; k < nc; ++k)
+        width[k] = cfg.per_code_width ? modelled_width(p.class_code[k]->code, half, cfg.grid_per_cu) : int(cfg.threads);
+    std::vector<int> widths(width);
+    std::sort(widths.begin(), widths.end(), std::greater<int>());
+    widths.erase(std::unique(widths.begin(), widths.end()), widths.end());
+    if (widths.size() > kMaxSegments) fail(Kind::Config, "too many launch widths");
+
+    std::vector<BatchClass> classes(nc);
     for (size_t k = 0; k < nc; ++k) {
         const NrCode& c = p.class_code[k]->code;
-        const int g = std::max(1, std::min(threads / int(c.z), kMaxGroup));
-        p.classes[k] = batch_class(c, p.class_set[k], p.device, g);
-        lds = std::max(lds, layered_lds_bytes(c.bg, int(c.z), g, p.half));
-        solo = solo && g == 1;
-        all384 = all384 && c.z == 384;
+        classes[k] = batch_class(c, p.class_set[k], p.device, group_of(int(c.z), width[k]));
     }
-    const int zc = solo && all384 && p.bg_mask != 3 ? 384 : 0;
-    const size_t shared_off = layered_shared_offset(lds);
-    lds = shared_off + kSharedBytes;
     std::vector<std::vector<uint32_t>> members(nc);
     for (size_t i = 0; i < p.count; ++i) members[p.cls_of[i]].push_back(uint32_t(i));
+
     struct Pending {
         BatchTask t;
         double cost;
         std::vector<uint32_t> ids;
     };
-    std::vector<Pending> pend;
-    for (size_t k = 0; k < nc; ++k) {
-        auto& m = members[k];
-        std::stable_sort(m.begin(), m.end(), [&](uint32_t x, uint32_t y) { return p.max_it[x] > p.max_it[y]; });
-        const int g = p.classes[k].g;
-        const NrCode& c = p.class_code[k]->code;
-        for (size_t s = 0; s < m.size(); s += size_t(g)) {
-            Pending q;
-            q.t = BatchTask{uint32_t(k), 0, uint32_t(std::min(m.size() - s, size_t(g))), 0};
-            q.ids.assign(m.begin() + long(s), m.begin() + long(s + q.t.count));
-            // a workgroup runs until its slowest codeword stops: limit x edges x Z per codeword
-            q.cost = double(std::max<uint16_t>(p.max_it[q.ids[0]], 1)) * double(c.edges_b) * double(c.z);
-            pend.push_back(std::move(q));
+    std::vector<BatchTask> tasks;
+    std::vector<uint32_t> order;
+    std::vector<PlanSegment> segs;
+    for (const int w : widths) {
+        PlanSegment sg;
+        sg.threads = w;
+        sg.first_task = tasks.size();
+        size_t lds = 0;
+        bool solo = true, all384 = true, nms = true;
+        int bgm = 0;
+        std::vector<Pending> pend;
+        for (size_t k = 0; k < nc; ++k) {
+            if (width[k] != w) continue;
+            const NrCode& c = p.class_code[k]->code;
+            const int g = classes[k].g;
+            lds = std::max(lds, layered_lds_bytes(c.bg, int(c.z), g, half));
+            solo = solo && g == 1;
+            all384 = all384 && c.z == 384;
+            nms = nms && p.class_set[k].rule == 1;
+            bgm |= c.bg;
+            ++sg.classes;
+            auto& m = members[k];
+            sg.codewords += uint32_t(m.size());
+            std::stable_sort(m.begin(), m.end(), [&](uint32_t x, uint32_t y) { return p.max_it[x] > p.max_it[y]; });
+            for (size_t s = 0; s < m.size(); s += size_t(g)) {
+                Pending q;
+                q.t = BatchTask{uint32_t(k), 0, uint32_t(std::min(m.size() - s, size_t(g))), 0};
+                q.ids.assign(m.begin() + long(s), m.begin() + long(s + q.t.count));
+                // a workgroup runs until its slowest codeword stops: limit x edges x Z per codeword
+                q.cost = double(std::max<uint16_t>(p.max_it[q.ids[0]], 1)) * double(c.edges_b) * double(c.z);
+                pend.push_back(std::move(q));
+            }
         }
+        std::stable_sort(pend.begin(), pend.end(), [](const Pending& x, const Pending& y) { return x.cost > y.cost; });
+        for (auto& q : pend) {
+            q.t.first = uint32_t(order.size());
+            tasks.push_back(q.t);
+            order.insert(order.end(), q.ids.begin(), q.ids.end());
+        }
+        sg.n_tasks = tasks.size() - sg.first_task;
+        // the normalized min-sum instantiation serves Z = 384 alone, one base graph
+        const int zc = solo && all384 && bgm != 3 ? 384 : 0;
+        nms = nms && zc == 384;
+        const size_t shared_off = layered_shared_offset(lds);
+        lds = shared_off + kSharedBytes;
+        int per_cu = 0;
+        check_hip(layered_occupancy(bgm, half, solo, zc, w, lds, &per_cu, nms), "occupancy");
+        per_cu = std::max(per_cu, 1);
+        if (cfg.grid_per_cu) per_cu = std::min(per_cu, int(cfg.grid_per_cu));
+        const int grid = int(std::min<size_t>(sg.n_tasks, size_t(per_cu) * size_t(cus)));
+        sg.launch = LayeredLaunch{bgm, half, solo, zc, w, std::max(grid, 1), lds, nms};
+        sg.args.shared_off = int32_t(shared_off);
+        segs.push_back(sg);
     }
-    std::stable_sort(pend.begin(), pend.end(), [](const Pending& x, const Pending& y) { return x.cost > y.cost; });
-    p.tasks.clear();
-    p.order.clear();
-    for (auto& q : pend) {
-        q.t.first = uint32_t(p.order.size());
-        p.tasks.push_back(q.t);
-        p.order.insert(p.order.end(), q.ids.begin(), q.ids.end());
-    }
-    int per_cu = 0, cus = 0;
-    check_hip(layered_occupancy(p.bg_mask, p.half, solo, zc, threads, lds, &per_cu), "occupancy");
-    check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p.device), "device attribute");
-    per_cu = std::max(per_cu, 1);
-    const int grid = int(std::min<size_t>(p.tasks.size(), size_t(per_cu) * size_t(std::max(cus, 1))));
-    p.threads = threads;
-    p.launch = LayeredLaunch{p.bg_mask, p.half, solo, zc, threads, std::max(grid, 1), lds};
 
-    // one device buffer: [queue | classes | tasks | order | llr_off | bit_off | max_it]
+    // one device buffer: [counters | classes | tasks | order | llr_off | bit_off | max_it]
     const size_t o_cls = 256, o_task = o_cls + up256(nc * sizeof(BatchClass));
-    const size_t o_order = o_task + up256(p.tasks.size() * sizeof(BatchTask));
+    const size_t o_order = o_task + up256(tasks.size() * sizeof(BatchTask));
     const size_t o_llr = o_order + up256(p.count * 4), o_bit = o_llr + up256(p.count * 8);
     const size_t o_it = o_bit + up256(p.count * 8), total = o_it + up256(p.count * 2);
     check_hip(hipDeviceSynchronize(), "synchronize");  // no launch still reads the old tables
@@ -237,37 +327,72 @@ void configure(cf_batch_plan& p, int threads) {
         p.dev_bytes = total;
     }
     std::vector<uint8_t> host(total, 0);
-    std::memcpy(host.data() + o_cls, p.classes.data(), nc * sizeof(BatchClass));
-    std::memcpy(host.data() + o_task, p.tasks.data(), p.tasks.size() * sizeof(BatchTask));
-    std::memcpy(host.data() + o_order, p.order.data(), p.count * 4);
+    std::memcpy(host.data() + o_cls, classes.data(), nc * sizeof(BatchClass));
+    std::memcpy(host.data() + o_task, tasks.data(), tasks.size() * sizeof(BatchTask));
+    std::memcpy(host.data() + o_order, order.data(), p.count * 4);
     std::memcpy(host.data() + o_llr, p.llr_off.data(), p.count * 8);
     std::memcpy(host.data() + o_bit, p.bit_off.data(), p.count * 8);
     std::memcpy(host.data() + o_it, p.max_it.data(), p.count * 2);
     check_hip(hipMemcpy(p.dev, host.data(), total, hipMemcpyHostToDevice), "copy plan");
     uint8_t* d = static_cast<uint8_t*>(p.dev);
-    LayeredArgs& a = p.args;
-    a = LayeredArgs{};
-    a.classes = reinterpret_cast<const BatchClass*>(d + o_cls);
-    a.tasks = reinterpret_cast<const BatchTask*>(d + o_task);
-    a.n_tasks = int32_t(p.tasks.size());
-    a.queue = reinterpret_cast<uint32_t*>(d);
-    a.order = reinterpret_cast<const uint32_t*>(d + o_order);
-    a.llr_off = reinterpret_cast<const uint64_t*>(d + o_llr);
-    a.bit_off = reinterpret_cast<const uint64_t*>(d + o_bit);
-    a.max_it = reinterpret_cast<const uint16_t*>(d + o_it);
-    a.shared_off = int32_t(shared_off);
+    p.counters = reinterpret_cast<uint64_t*>(d);
+    for (size_t s = 0; s < segs.size(); ++s) {
+        LayeredArgs& a = segs[s].args;
+        const int32_t shared_off = a.shared_off;
+        a = LayeredArgs{};
+        a.classes = reinterpret_cast<const BatchClass*>(d + o_cls);
+        a.tasks = reinterpret_cast<const BatchTask*>(d + o_task) + segs[s].first_task;
+        a.n_tasks = int32_t(segs[s].n_tasks);
+        a.queue = reinterpret_cast<uint32_t*>(p.counters + s);
+        a.order = reinterpret_cast<const uint32_t*>(d + o_order);
+        a.llr_off = reinterpret_cast<const uint64_t*>(d + o_llr);
+        a.bit_off = reinterpret_cast<const uint64_t*>(d + o_bit);
+        a.max_it = reinterpret_cast<const uint16_t*>(d + o_it);
+        a.shared_off = shared_off;
+    }
+    // the side streams and events of an overlapped run (default priority)
+    if (cfg.overlap)
+        while (p.side.size() + 1 < segs.size()) {
+            if (!p.fork) check_hip(hipEventCreateWithFlags(&p.fork, hipEventDisableTiming), "event");
+            hipStream_t q = nullptr;
+            check_hip(hipStreamCreateWithFlags(&q, hipStreamNonBlocking), "hipStreamCreate");
+            p.side.push_back(q);
+            hipEvent_t e = nullptr;
+            check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+            p.joins.push_back(e);
+        }
+    p.cfg = cfg;
+    if (cfg.per_code_width) p.cfg.threads = 0;
+    p.half = half;
+    p.class_width = std::move(width);
+    p.classes = std::move(classes);
+    p.tasks = std::move(tasks);
+    p.order = std::move(order);
+    p.segs = std::move(segs);
 }
 
-// the counter fill and the decode launch
+// the counter fill and the decode launches
 void enqueue(cf_batch_plan& p, const float* llr, uint8_t* bits, uint32_t* its, uint8_t* ok, hipStream_t s,
              bool set_attribute) {
-    LayeredArgs a = p.args;
-    a.llr = llr;
-    a.bits = bits;
-    a.iterations = its;
-    a.ok = ok;
-    check_hip(launch_zero(reinterpret_cast<uint64_t*>(a.queue), 1, s), "zero launch");
-    check_hip(launch_layered(a, p.launch, s, set_attribute), "batch decode launch");
+    check_hip(launch_zero(p.counters, int(p.segs.size()), s), "zero launch");
+    const bool overlap = p.cfg.overlap && p.segs.size() > 1;
+    if (overlap) check_hip(hipEventRecord(p.fork, s), "fork");
+    for (size_t i = 0; i < p.segs.size(); ++i) {
+        LayeredArgs a = p.segs[i].args;
+        a.llr = llr;
+        a.bits = bits;
+        a.iterations = its;
+        a.ok = ok;
+        hipStream_t q = s;
+        if (overlap && i > 0) {
+            q = p.side[i - 1];
+            check_hip(hipStreamWaitEvent(q, p.fork, 0), "fork");
+        }
+        check_hip(launch_layered(a, p.segs[i].launch, q, set_attribute), "batch decode launch");
+        if (q != s) check_hip(hipEventRecord(p.joins[i - 1], q), "join");
+    }
+    if (overlap)
+        for (size_t i = 1; i < p.segs.size(); ++i) check_hip(hipStreamWaitEvent(s, p.joins[i - 1], 0), "join");
 }
 
 void run_plan(cf_batch_plan& p, const float* llr, uint8_t* bits, uint32_t* its, uint8_t* ok, bool use_graph,
@@ -282,7 +407,7 @@ void run_plan(cf_batch_plan& p, const float* llr, uint8_t* bits, uint32_t* its, 
         p.drop_graph();
         if (!p.capture_stream) check_hip(hipStreamCreate(&p.capture_stream), "hipStreamCreate");
         // kernel attributes are set outside the capture
-        check_hip(layered_prepare(p.launch), "kernel attribute");
+        for (const PlanSegment& sg : p.segs) check_hip(layered_prepare(sg.launch), "kernel attribute");
         check_hip(hipStreamBeginCapture(p.capture_stream, hipStreamCaptureModeThreadLocal), "begin capture");
         try {
             enqueue(p, llr, bits, its, ok, p.capture_stream, false);
@@ -302,6 +427,19 @@ void run_plan(cf_batch_plan& p, const float* llr, uint8_t* bits, uint32_t* its, 
     check_hip(hipGraphLaunch(p.exec, s), "graph launch");
 }
 
+// What a configuration launches: per class its width, per launch its kernel and grid.
+std::vector<int64_t> launch_signature(const cf_batch_plan& p) {
+    std::vector<int64_t> v(p.class_width.begin(), p.class_width.end());
+    v.push_back(p.half ? 1 : 0);
+    v.push_back(p.cfg.overlap && p.segs.size() > 1 ? 1 : 0);
+    for (const PlanSegment& sg : p.segs) {
+        v.push_back(sg.threads);
+        v.push_back(sg.launch.grid);
+        v.push_back(int64_t(sg.launch.lds));
+    }
+    return v;
+}
+
 cf_batch_plan* create_plan(const cf_code* const* codes, const uint64_t* llr_offsets, const uint32_t* max_iterations,
                            size_t count, int msg_type) {
     if (!codes || !llr_offsets) fail(Kind::Config, "null code list or offsets");
@@ -311,6 +449,7 @@ cf_batch_plan* create_plan(const cf_code* const* codes, const uint64_t* llr_offs
     auto p = std::make_unique<cf_batch_plan>();
     p->device = current_device();
     p->half = msg_type == CF_MSG_F16;
+    p->cfg.msg_type = msg_type;
     p->count = count;
     p->cls_of.resize(count);
     p->max_it.resize(count);
@@ -345,7 +484,8 @@ cf_batch_plan* create_plan(const cf_code* const* codes, const uint64_t* llr_offs
         p->bits_total += c.k_info;
     }
     p->llr_total = count ? llr_offsets[count] : 0;
-    if (count) configure(*p, std::min(kMaxThreads, round64(p->max_z)));
+    p->cfg.threads = uint32_t(std::min(kMaxThreads, round64(p->max_z)));
+    if (count) configure(*p, p->cfg);
     return p.release();
 }
 
@@ -366,15 +506,17 @@ void cf_batch_plan_destroy(cf_batch_plan* plan) { delete plan; }
 cf_status cf_batch_plan_dims_get(const cf_batch_plan* plan, cf_batch_plan_dims* out) {
     return guarded([&] {
         if (!plan || !out) fail(Kind::Config, "null plan");
+        const PlanSegment* s0 = plan->segs.empty() ? nullptr : &plan->segs[0];
         *out = cf_batch_plan_dims{plan->llr_total,
                                   plan->bits_total,
                                   uint32_t(plan->count),
                                   uint32_t(plan->class_code.size()),
                                   uint32_t(plan->tasks.size()),
-                                  uint32_t(plan->threads),
-                                  uint32_t(plan->launch.grid),
-                                  uint32_t(plan->launch.lds),
-                                  plan->half ? CF_MSG_F16 : CF_MSG_F32};
+                                  s0 ? uint32_t(s0->threads) : plan->cfg.threads,
+                                  s0 ? uint32_t(s0->launch.grid) : 0u,
+                                  s0 ? uint32_t(s0->launch.lds) : 0u,
+                                  plan->half ? CF_MSG_F16 : CF_MSG_F32,
+                                  uint32_t(plan->segs.size())};
     });
 }
 
@@ -382,27 +524,83 @@ cf_status cf_batch_plan_set_threads(cf_batch_plan* plan, uint32_t threads) {
     return guarded([&] {
         if (!plan) fail(Kind::Config, "null plan");
         std::lock_guard<std::mutex> l(plan->mu);
-        if (plan->count) configure(*plan, int(threads));
+        cf_batch_plan_config c = plan->cfg;
+        c.threads = threads;
+        c.per_code_width = 0;
+        check_config(*plan, c);
+        if (plan->count) configure(*plan, c);
+        else plan->cfg = c;
+    });
+}
+
+cf_status cf_batch_plan_config_get(const cf_batch_plan* plan, cf_batch_plan_config* out) {
+    return guarded([&] {
+        if (!plan || !out) fail(Kind::Config, "null plan");
+        *out = plan->cfg;
+    });
+}
+
+cf_status cf_batch_plan_configure(cf_batch_plan* plan, const cf_batch_plan_config* config) {
+    return guarded([&] {
+        if (!plan || !config) fail(Kind::Config, "null plan or configuration");
+        std::lock_guard<std::mutex> l(plan->mu);
+        check_config(*plan, *config);
+        if (plan->count) configure(*plan, *config);
+        else {
+            plan->cfg = *config;
+            plan->half = config->msg_type == CF_MSG_F16;
+        }
+    });
+}
+
+cf_status cf_batch_plan_segment_get(const cf_batch_plan* plan, uint32_t index, cf_batch_plan_segment* out) {
+    return guarded([&] {
+        if (!plan || !out) fail(Kind::Config, "null plan");
+        if (index >= plan->segs.size())
+            fail(Kind::Config, "launch " + std::to_string(index) + " of " + std::to_string(plan->segs.size()));
+        const PlanSegment& s = plan->segs[index];
+        *out = cf_batch_plan_segment{uint32_t(s.threads), uint32_t(s.launch.grid), uint32_t(s.launch.lds),
+                                     s.classes,           uint32_t(s.n_tasks),     s.codewords};
     });
 }
 
 cf_status cf_batch_plan_tune(cf_batch_plan* plan, const float* d_llr, uint8_t* d_bits, uint32_t* d_iterations,
-                             uint8_t* d_ok, uint32_t repeats, double* best_ms, void* stream) {
+                             uint8_t* d_ok, uint32_t repeats, uint32_t flags, cf_batch_plan_trial* trials,
+                             size_t trials_cap, size_t* trials_count, double* best_ms, void* stream) {
     return guarded([&] {
         if (!plan) fail(Kind::Config, "null plan");
+        if (flags & ~CF_TUNE_STORAGE) fail(Kind::Config, "unknown tuning flags " + std::to_string(flags));
         std::lock_guard<std::mutex> l(plan->mu);
+        if (trials_count) *trials_count = 0;
         if (plan->count == 0) return;
         hipStream_t s = static_cast<hipStream_t>(stream);
         repeats = std::max<uint32_t>(repeats, 1);
         hipEvent_t e0, e1;
         check_hip(hipEventCreate(&e0), "event");
-        check_hip(hipEventCreate(&e1), "event");
         std::unique_ptr<void, void (*)(void*)> g0(e0, [](void* e) { (void)hipEventDestroy(hipEvent_t(e)); });
+        check_hip(hipEventCreate(&e1), "event");
         std::unique_ptr<void, void (*)(void*)> g1(e1, [](void* e) { (void)hipEventDestroy(hipEvent_t(e)); });
-        int best = 0;
-        float best_t = 0.0f;
-        for (int t = round64(plan->max_z); t <= kMaxThreads; t += 64) {
-            configure(*plan, t);
+
+        std::vector<int> storages{plan->cfg.msg_type};
+        if (flags & CF_TUNE_STORAGE) storages.push_back(plan->cfg.msg_type == CF_MSG_F32 ? CF_MSG_F16 : CF_MSG_F32);
+        std::vector<cf_batch_plan_config> cands;
+        for (const int m : storages)
+            for (const uint32_t per_cu : {0u, 1u}) {
+                for (int t = round64(plan->max_z); t <= kMaxThreads; t += 64)
+                    cands.push_back(cf_batch_plan_config{uint32_t(t), 0u, per_cu, m, 0u});
+                cands.push_back(cf_batch_plan_config{0u, 1u, per_cu, m, 0u});
+                cands.push_back(cf_batch_plan_config{0u, 1u, per_cu, m, 1u});
+            }
+        std::vector<std::vector<int64_t>> seen;
+        size_t n = 0;
+        bool have = false;
+        cf_batch_plan_config best{};
+        double best_t = 0.0;
+        for (const cf_batch_plan_config& c : cands) {
+            configure(*plan, c);
+            std::vector<int64_t> sig = launch_signature(*plan);
+            if (std::find(seen.begin(), seen.end(), sig) != seen.end()) continue;
+            seen.push_back(std::move(sig));
             run_plan(*plan, d_llr, d_bits, d_iterations, d_ok, false, s);  // warm-up
             check_hip(hipEventRecord(e0, s), "event");
             for (uint32_t i = 0; i < repeats; ++i) run_plan(*plan, d_llr, d_bits, d_iterations, d_ok, false, s);
@@ -410,14 +608,20 @@ cf_status cf_batch_plan_tune(cf_batch_plan* plan, const float* d_llr, uint8_t* d
             check_hip(hipEventSynchronize(e1), "event");
             float ms = 0.0f;
             check_hip(hipEventElapsedTime(&ms, e0, e1), "event");
-            ms /= float(repeats);
-            if (best == 0 || ms < best_t) {
-                best = t;
-                best_t = ms;
+            const double t = double(ms) / double(repeats);
+            if (trials && n < trials_cap) trials[n] = cf_batch_plan_trial{plan->cfg, uint32_t(plan->segs.size()), t};
+            ++n;
+            if (!have || t < best_t) {
+                have = true;
+                best = plan->cfg;
+                best_t = t;
             }
         }
         configure(*plan, best);
-        if (best_ms) *best_ms = double(best_t);
+        run_plan(*plan, d_llr, d_bits, d_iterations, d_ok, false, s);
+        check_hip(hipStreamSynchronize(s), "tuned decode");
+        if (trials_count) *trials_count = n;
+        if (best_ms) *best_ms = best_t;
     });
 }
 

@@ -36,7 +36,10 @@ def main():
     p.add_argument("--half", action="store_true", help="FP16 message storage")
     p.add_argument("--uneven", type=int, default=0, help="codewords of the uneven workload")
     p.add_argument("--graph", action="store_true", help="uneven: replay a captured HIP graph")
-    p.add_argument("--tune", action="store_true", help="uneven: autotune the workgroup width first")
+    p.add_argument("--tune", action="store_true", help="uneven: autotune the plan's configuration first")
+    p.add_argument("--tune-storage", action="store_true", help="uneven: let the autotuner change the storage too")
+    p.add_argument("--per-code-width", action="store_true", help="uneven: each code at its own width")
+    p.add_argument("--overlap", action="store_true", help="uneven: the launches of a run side by side")
     p.add_argument("--dvbs2", type=int, default=0,
                    help="codewords of the DVB-S2 short rate-1/2 code (layered sum-product, FP32, 50 it)")
     p.add_argument("--ebn0", type=float, default=0.9, help="dvbs2: Eb/N0 in dB")
@@ -206,16 +209,22 @@ def uneven(a, torch, F):
     flat = torch.cat(per).contiguous()
     plan = F.BatchPlan(cl, offsets, limits, half=a.half)
     out = plan.outputs()
+    if a.per_code_width:
+        plan.configure(per_code_width=True, overlap=a.overlap)
     if a.tune:
-        best, _ = plan.tune(flat, out, repeats=3)
-        print(f"tuned: {plan.dims.threads} lanes ({best:.3f} ms)", flush=True)
+        best, _ = plan.tune(flat, out, repeats=3, storage=a.tune_storage)
+        for t in plan.trials:
+            print(f"  trial {t.config!r}: {t.segments} launch(es), {t.ms:.3f} ms", flush=True)
+        print(f"tuned: {plan.config!r} ({best:.3f} ms)", flush=True)
     plan.run(flat, out=out, graph=a.graph)
     ms = timed(lambda: plan.run(flat, out=out, graph=a.graph), a.steps, torch)
     d = plan.dims
     its = out[1].double()
-    print(f"uneven{' fp16' if a.half else ''}{' graph' if a.graph else ''}: {n} codewords, {d.classes} codes, "
-          f"{d.tasks} tasks, grid {d.grid} x {d.threads} lanes, LDS {d.lds_bytes} B, mean iterations "
-          f"{its.mean().item():.3f}, {ms:.3f} ms per launch, {n / ms * 1e3:.0f} codewords/s, "
+    launches = ", ".join(f"{sg.grid} x {sg.threads} lanes ({sg.classes} codes, {sg.tasks} tasks, LDS {sg.lds_bytes} B)"
+                         for sg in plan.segments())
+    print(f"uneven{' fp16' if d.msg_type == F.MSG_F16 else ''}{' graph' if a.graph else ''}: {n} codewords, "
+          f"{d.classes} codes, {d.tasks} tasks, launches: {launches}; mean iterations "
+          f"{its.mean().item():.3f}, {ms:.3f} ms per run, {n / ms * 1e3:.0f} codewords/s, "
           f"{d.bits_total / ms / 1e6:.3f} Gbit/s", flush=True)
 
 

@@ -152,9 +152,12 @@ cf_status cf_decode_batch_static(const cf_code* code, int msg_type, const void* 
    E_i = llr_offsets[i+1] - llr_offsets[i] f32 LLRs d_llr[llr_offsets[i] ..]; its K'_i
    decisions go to d_bits at the sum of the K'_j of the codewords before it; d_iterations
    and d_ok (may be NULL) get one entry per codeword. max_iterations: per codeword, or NULL
-   for each code's own limit. One launch decodes the whole list: workgroups take tasks (up
-   to floor(threads / Z) codewords of one code) from a counter, the most work first.
-   msg_type: CF_MSG_F32 or CF_MSG_F16. A plan is used by one thread at a time. */
+   for each code's own limit. A launch decodes the codewords of its codes: workgroups take
+   tasks (up to floor(threads / Z) codewords of one code) from a counter, the most work
+   first. By default one launch takes every code at the width of the largest Z rounded up to
+   a multiple of 64 (a configuration may give each code its own width, one launch per width,
+   in one stream). msg_type: CF_MSG_F32 or CF_MSG_F16 (the initial storage). A plan is used
+   by one thread at a time. */
 typedef struct cf_batch_plan cf_batch_plan;
 
 typedef struct {
@@ -165,7 +168,39 @@ typedef struct {
     uint32_t grid;          /* workgroups of the launch (resident ones: the launch is persistent) */
     uint32_t lds_bytes;     /* dynamic LDS per workgroup */
     int msg_type;
+    uint32_t segments;      /* launches of a run; threads, grid and lds_bytes are the first's */
 } cf_batch_plan_dims;
+
+/* How a plan runs. The decisions, iteration counts and verdicts do not depend on threads,
+   per_code_width or grid_per_cu; msg_type chooses the arithmetic of the storage (FP32, or
+   binary16 posteriors and messages). */
+typedef struct {
+    uint32_t threads;         /* one launch at this width: a multiple of 64 from the largest Z
+                                 (rounded up) to 384; ignored with per_code_width */
+    uint32_t per_code_width;  /* 1: each code at the width (a multiple of 64 from its Z rounded
+                                 up to 384) with the most codeword rows per busy SIMD at the
+                                 device's occupancy, one launch per width, the widest first */
+    uint32_t grid_per_cu;     /* resident workgroups per CU at most (0: as many as fit) */
+    int msg_type;             /* CF_MSG_F32 or CF_MSG_F16 */
+    uint32_t overlap;         /* 1: the launches of a run go on parallel streams forked from
+                                 and joined to the caller's, so that one launch's workgroups
+                                 take the CUs another's leave; 0: one after another */
+} cf_batch_plan_config;
+
+/* One launch of a plan's run. */
+typedef struct {
+    uint32_t threads, grid, lds_bytes;
+    uint32_t classes, tasks, codewords;
+} cf_batch_plan_segment;
+
+/* A configuration the autotuner timed and its mean time per run. */
+typedef struct {
+    cf_batch_plan_config config;
+    uint32_t segments;
+    double ms;
+} cf_batch_plan_trial;
+
+#define CF_TUNE_STORAGE 1u /* the autotuner also tries the other storage type */
 
 cf_status cf_batch_plan_create(const cf_code* const* codes, const uint64_t* llr_offsets,
                                const uint32_t* max_iterations, size_t count, int msg_type,
@@ -173,19 +208,31 @@ cf_status cf_batch_plan_create(const cf_code* const* codes, const uint64_t* llr_
 void cf_batch_plan_destroy(cf_batch_plan* plan);
 cf_status cf_batch_plan_dims_get(const cf_batch_plan* plan, cf_batch_plan_dims* out);
 
-/* The workgroup width: a multiple of 64 from the largest Z (rounded up) to 384. The
-   results do not depend on it. Waits for the device. */
+/* One launch at this width (the other settings kept): a multiple of 64 from the largest Z
+   (rounded up) to 384. The results do not depend on it. Waits for the device. */
 cf_status cf_batch_plan_set_threads(cf_batch_plan* plan, uint32_t threads);
 
-/* The autotuner: times one launch of every admissible width on these buffers (repeats
-   times each, after a warm-up) and keeps the fastest; *best_ms (may be NULL) gets its
-   time. The outputs hold a complete decode afterwards. */
+/* The configuration as set, and a new one (waits for the device). */
+cf_status cf_batch_plan_config_get(const cf_batch_plan* plan, cf_batch_plan_config* out);
+cf_status cf_batch_plan_configure(cf_batch_plan* plan, const cf_batch_plan_config* config);
+/* Launch `index` (0 .. segments - 1) of a run. */
+cf_status cf_batch_plan_segment_get(const cf_batch_plan* plan, uint32_t index, cf_batch_plan_segment* out);
+
+/* The autotuner: times a run of every candidate configuration on these buffers (repeats
+   times each, after a warm-up) and keeps the fastest. The candidates: one launch at every
+   admissible width and the per-code widths (their launches one after another and side by
+   side), each with as many resident workgroups per CU as fit and with one, on the plan's
+   storage (flags CF_TUNE_STORAGE: on both); configurations that launch the same are timed
+   once. The first min(trials_cap, candidates) trials go to
+   `trials` (may be NULL) and their number to *trials_count, the chosen time to *best_ms
+   (both may be NULL). The outputs hold a decode of the chosen configuration afterwards. */
 cf_status cf_batch_plan_tune(cf_batch_plan* plan, const float* d_llr, uint8_t* d_bits,
-                             uint32_t* d_iterations, uint8_t* d_ok, uint32_t repeats,
+                             uint32_t* d_iterations, uint8_t* d_ok, uint32_t repeats, uint32_t flags,
+                             cf_batch_plan_trial* trials, size_t trials_cap, size_t* trials_count,
                              double* best_ms, void* stream);
 
-/* Decode on `stream`. With use_graph the counter fill and the launch are captured once as a
-   HIP graph and replayed while the buffers stay the same. */
+/* Decode on `stream`. With use_graph the counter fill and the launches are captured once as
+   a HIP graph and replayed while the buffers and the configuration stay the same. */
 cf_status cf_batch_plan_run(cf_batch_plan* plan, const float* d_llr, uint8_t* d_bits,
                             uint32_t* d_iterations, uint8_t* d_ok, int use_graph, void* stream);
 

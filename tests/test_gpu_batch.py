@@ -12,6 +12,8 @@ launch, through the C ABI.
   results do not depend on the workgroup width the autotuner picks, and a replayed HIP
   graph gives the eager results on every replay.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -239,8 +241,14 @@ def test_width_autotuner_and_graph_replays():
         plan.set_threads(w)
         got = plan.run(flat)
         assert all(torch.equal(a, b) for a, b in zip(got, eager)), f"width {w}"
+    plan.configure(per_code_width=True)
+    got = plan.run(flat)
+    assert all(torch.equal(a, b) for a, b in zip(got, eager)), "per-code widths"
     ms, got = plan.tune(flat, repeats=2)
-    assert ms > 0 and plan.dims.threads in (128, 192, 256, 320, 384)
+    print(f"  {len(plan.trials)} trials, chosen {plan.config!r}: {ms:.3f} ms", flush=True)
+    assert ms > 0 and len(plan.trials) >= 6
+    assert any(t.ms == ms for t in plan.trials)
+    assert plan.config.msg_type == F.MSG_F32
     assert all(torch.equal(a, b) for a, b in zip(got, eager))
     out = plan.outputs()
     for rep in range(3):
@@ -252,6 +260,72 @@ def test_width_autotuner_and_graph_replays():
         assert all(torch.equal(a, b) for a, b in zip(out, eager)), f"replay {rep}"
         _ = (flat * 2.0).sum().item()  # eager work between the replays
     check_against_oracle(codes, desc, fulls, eager, False, idx=range(0, 900, 7))
+
+
+def test_plan_configurations_bit_identical():
+    """Z <= 64: one launch at every width from 64 to 384, per-code widths (several launches in
+    one run) and one resident workgroup per CU all give the same decisions, iterations and
+    verdicts, equal to the oracle's."""
+    print("\n[uneven batch] configurations, Z <= 64", flush=True)
+    small = [z for z in LIFTS if z <= 64]
+    codes, desc, flat, offsets, fulls = uneven_workload(700, seed=9, lifts=small)
+    plan = F.BatchPlan(codes, offsets, desc[2])
+    assert plan.dims.threads == 64 and plan.dims.segments == 1
+    eager = [t.clone() for t in plan.run(flat)]
+    for w in (64, 128, 192, 256, 320, 384):
+        for per_cu in (0, 1):
+            plan.configure(threads=w, per_code_width=False, grid_per_cu=per_cu)
+            got = plan.run(flat)
+            assert all(torch.equal(a, b) for a, b in zip(got, eager)), f"width {w}, {per_cu} per CU"
+    plan.configure(per_code_width=True, grid_per_cu=0)
+    segs = plan.segments()
+    print("  per-code widths: " + ", ".join(f"{s.threads} lanes x {s.classes} codes" for s in segs), flush=True)
+    assert sum(s.codewords for s in segs) == 700 and sum(s.tasks for s in segs) == plan.dims.tasks
+    assert [s.threads for s in segs] == sorted({s.threads for s in segs}, reverse=True)
+    got = plan.run(flat)
+    assert all(torch.equal(a, b) for a, b in zip(got, eager)), "per-code widths"
+    # the launches side by side, eagerly and through a replayed graph
+    plan.configure(overlap=True)
+    assert plan.config.overlap == 1 and plan.dims.segments == len(segs)
+    got = plan.run(flat)
+    assert all(torch.equal(a, b) for a, b in zip(got, eager)), "overlapped launches"
+    out = plan.outputs()
+    for rep in range(3):
+        for t in out:
+            t.fill_(0x3C if t.dtype == torch.uint8 else -1)
+        torch.cuda.synchronize()
+        plan.run(flat, out=out, graph=True)
+        torch.cuda.synchronize()
+        assert all(torch.equal(a, b) for a, b in zip(out, eager)), f"overlapped replay {rep}"
+        _ = (flat * 3.0).sum().item()
+    check_against_oracle(codes, desc, fulls, eager, False, idx=range(0, 700, 5))
+
+
+def test_tuner_on_20000_codewords():
+    """The 20 000-codeword plan with FP16 storage: the tuner times at least two candidates
+    and keeps results bit-identical; allowed to change the storage, its results equal a plan
+    made with the storage it chose."""
+    print("\n[uneven batch] tuner, 20000 codewords", flush=True)
+    codes, desc, flat, offsets, fulls = uneven_workload(20000, seed=7)
+    plan = F.BatchPlan(codes, offsets, desc[2], half=True)
+    eager = [t.clone() for t in plan.run(flat)]
+    ms, got = plan.tune(flat, repeats=2)
+    for t in plan.trials:
+        print(f"  trial {t.config!r}: {t.segments} launch(es), {t.ms:.3f} ms", flush=True)
+    print(f"  chosen {plan.config!r}: {ms:.3f} ms", flush=True)
+    assert len(plan.trials) >= 2 and ms == min(t.ms for t in plan.trials)
+    assert {t.config.overlap for t in plan.trials if t.segments > 1} == {0, 1}
+    assert plan.config.msg_type == F.MSG_F16
+    assert all(torch.equal(a, b) for a, b in zip(got, eager))
+    assert all(torch.equal(a, b) for a, b in zip(plan.run(flat), eager))
+    ms2, got = plan.tune(flat, repeats=2, storage=True)
+    chosen = plan.config
+    print(f"  with storage: {len(plan.trials)} trials, chosen {chosen!r}: {ms2:.3f} ms", flush=True)
+    assert {t.config.msg_type for t in plan.trials} == {F.MSG_F32, F.MSG_F16}
+    half = chosen.msg_type == F.MSG_F16
+    direct = F.BatchPlan(codes, offsets, desc[2], half=half).run(flat)
+    assert all(torch.equal(a, b) for a, b in zip(got, direct))
+    check_against_oracle(codes, desc, fulls, got, half, idx=range(0, 20000, 41))
 
 
 def test_plan_errors():
@@ -269,6 +343,16 @@ def test_plan_errors():
     plan = F.BatchPlan([a, b], np.array([0, a.dims.e, a.dims.e + b.dims.e], np.uint64))
     with pytest.raises(F.ComfecError):
         plan.set_threads(96)
+    with pytest.raises(F.ComfecError) as e:
+        plan.configure(threads=0, per_code_width=False)
+    assert e.value.kind == "Config"
+    plan.configure(per_code_width=True)
+    with pytest.raises(F.ComfecError) as e:
+        plan.configure(per_code_width=2)
+    assert e.value.kind == "Config"
+    c = plan.config
+    c.msg_type = F.MSG_F64
+    assert F.lib().cf_batch_plan_configure(plan._h, C.byref(c)) == F.CF_CONFIG
 
 
 @pytest.mark.parametrize("k,e,rv,qm", [(500, 1200, 2, 4), (60, 1000, 1, 2), (1000, 1500, 3, 6),
