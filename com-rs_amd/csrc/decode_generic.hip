@@ -14,6 +14,7 @@
 #include <algorithm>
 
 #include "kernels.hpp"
+#include "sizing.hpp"
 
 namespace comfec {
 
@@ -731,23 +732,14 @@ hipError_t launch_rows(const GenericArgs<T>& a0, Workspace& ws, hipStream_t stre
     GenericArgs<T> a = a0;
     const DevCode& c = a.code;
     const size_t per_lane = (size_t(c.n) + size_t(c.edges_b)) * sizeof(T);
-    int64_t lanes = int64_t(kRowsScratchBytes / per_lane) & ~int64_t(63);
-    if (lanes < 64) lanes = 64;
-    if (ws.fixed) {
-        // a buffer given whole: as many lanes as it holds
-        if (ws.bytes < per_lane) return hipErrorOutOfMemory;
-        lanes = std::min<int64_t>(lanes, int64_t(ws.bytes / per_lane));
-    }
-    if (lanes > a.count) lanes = a.count;
-    if (lanes < 1) return hipSuccess;
-    // fewer lanes (each then decodes more codewords) when the device cannot hold them all
-    T* scratch = static_cast<T*>(ws.get(size_t(lanes) * per_lane));
-    while (!scratch && lanes > 64) {
-        lanes = std::max<int64_t>(64, (lanes / 2) & ~int64_t(63));
-        scratch = static_cast<T*>(ws.get(size_t(lanes) * per_lane));
-    }
-    if (!scratch) return hipErrorOutOfMemory;
-    a.scratch = scratch;
+    // fewer lanes (each then decodes more codewords) when the device cannot hold them all;
+    // a buffer given whole: as many lanes as it holds
+    const Sizing sz = size_scratch(a.count, per_lane, kRowsScratchBytes, 64, 64, ws.fixed, ws.bytes,
+                                   [&](size_t bytes) { return ws.get(bytes); });
+    if (sz.status == Sizing::kNothing) return hipSuccess;
+    if (sz.status == Sizing::kOutOfMemory) return hipErrorOutOfMemory;
+    const int64_t lanes = sz.units;
+    a.scratch = static_cast<T*>(sz.scratch);
     a.scratch_stride = 0;
     // one wavefront per workgroup, so that few codewords still spread over many CUs
     const int threads = 64;
@@ -762,21 +754,13 @@ hipError_t launch_groups(const GenericArgs<T>& a0, Workspace& ws, hipStream_t st
     const DevCode& c = a.code;
     // posteriors, messages and one value the lanes past a check's edges write
     const size_t per_slot = (size_t(c.n) + size_t(c.edges_b) + 1) * sizeof(T);
-    int64_t slots = int64_t(kRowsScratchBytes / per_slot);
-    if (ws.fixed) {
-        if (ws.bytes < per_slot) return hipErrorOutOfMemory;
-        slots = std::min<int64_t>(slots, int64_t(ws.bytes / per_slot));
-    }
-    if (slots > a.count) slots = a.count;
-    if (slots < 1) return hipSuccess;
     // fewer groups (each then decodes more codewords) when the device cannot hold them all
-    T* scratch = static_cast<T*>(ws.get(size_t(slots) * per_slot));
-    while (!scratch && slots > 64) {
-        slots /= 2;
-        scratch = static_cast<T*>(ws.get(size_t(slots) * per_slot));
-    }
-    if (!scratch) return hipErrorOutOfMemory;
-    a.scratch = scratch;
+    const Sizing sz = size_scratch(a.count, per_slot, kRowsScratchBytes, 1, 1, ws.fixed, ws.bytes,
+                                   [&](size_t bytes) { return ws.get(bytes); });
+    if (sz.status == Sizing::kNothing) return hipSuccess;
+    if (sz.status == Sizing::kOutOfMemory) return hipErrorOutOfMemory;
+    const int64_t slots = sz.units;
+    a.scratch = static_cast<T*>(sz.scratch);
     a.scratch_stride = 0;
     const unsigned blocks = unsigned((slots * G + 63) / 64);
     hipLaunchKernelGGL((decode_groups_kernel<T, G>), dim3(blocks), dim3(64), 0, stream, a, slots);
