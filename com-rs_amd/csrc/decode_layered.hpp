@@ -46,6 +46,9 @@ namespace comfec {
 
 namespace layered {
 
+// two 16-bit lanes of one register (packed integer arithmetic)
+typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
+
 template <int BG>
 struct Graph;
 template <>
@@ -245,9 +248,9 @@ struct Messages<ROWS, true> {
 // funnel shift) and one index whose magnitude is g(m2); a message is rebuilt as that
 // magnitude with its sign bit replaced by the stored one, which is the output bit for bit.
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, int I>
-__device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post, int r, uint32_t gofs, int z,
-                                      const uint32_t* __restrict__ shift, Messages<Graph<BG>::rows, HALF>& msg,
-                                      float a, float b) {
+__device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post, int r, uint32_t gofs, uint32_t rbp,
+                                      int z, const uint32_t* __restrict__ shift,
+                                      Messages<Graph<BG>::rows, HALF>& msg, float a, float b) {
     using S = Store<HALF>;
     constexpr int e0 = row_begin<BG>(I);
     constexpr int deg = row_begin<BG>(I + 1) - e0;
@@ -261,7 +264,7 @@ __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post
         // the shifts are constants too, so the row offset is what keeps the addresses (all
         // 316 of them loop-invariant) from being computed once and held in registers
         z = ZC;
-        asm volatile("" : "+v"(rb));
+        asm volatile("" : "+v"(rb), "+v"(rbp));
     } else {
         asm volatile("" : "+s"(z));
     }
@@ -291,8 +294,15 @@ __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post
         const int e = e0 + q;
         if constexpr (ZC != 0) {
             const uint32_t s_q = shift_of<BG, ZC>(e) * es;
-            const uint32_t t = rb + s_q;
-            const uint32_t rr = s_q == 0 ? t : min(t, t - uint32_t(ZC) * es);
+            uint32_t rr = rb;
+            if (s_q != 0) {
+                // rbp holds rb and rb - Z (mod 2^16) in its halves, so one 32-bit add of s to
+                // both halves gives t = rb + s and t - Z at once (t < 2 Z * 4 <= 3072: no carry
+                // from the low half; the high half's carry leaves the word), and their
+                // unsigned 16-bit min is t mod Z
+                const u16x2 t = __builtin_bit_cast(u16x2, rbp + (s_q | (s_q << 16)));
+                rr = min(uint32_t(t.x), uint32_t(t.y));
+            }
             return int(rr) + int(Graph<BG>::t[e].j) * ZC * int(es);
         } else {
             const uint32_t t = uint32_t(r) + sq[ZC != 0 ? 0 : q];
@@ -372,7 +382,8 @@ __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post
 }
 
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, int I>
-__device__ __forceinline__ void sweep(typename Store<HALF>::T* post, int r, uint32_t gofs, int z, bool active,
+__device__ __forceinline__ void sweep(typename Store<HALF>::T* post, int r, uint32_t gofs, uint32_t rbp, int z,
+                                      bool active,
                                       const uint32_t* shift, Messages<Graph<BG>::rows, HALF>& msg, const float* ab,
                                       float a, float b) {
     if constexpr (I < Graph<BG>::rows) {
@@ -383,8 +394,8 @@ __device__ __forceinline__ void sweep(typename Store<HALF>::T* post, int r, uint
         if constexpr (Sched<BG>::s.barrier[I]) __syncthreads();
         // (the branch also bounds the scheduler's reach: without it the compiler interleaves
         // layers and spills hundreds of registers)
-        if (active) layer<BG, HALF, SOLO, ZC, NMS, I>(post, r, gofs, z, shift, msg, a, b);
-        sweep<BG, HALF, SOLO, ZC, NMS, I + 1>(post, r, gofs, z, active, shift, msg, ab, a, b);
+        if (active) layer<BG, HALF, SOLO, ZC, NMS, I>(post, r, gofs, rbp, z, shift, msg, a, b);
+        sweep<BG, HALF, SOLO, ZC, NMS, I + 1>(post, r, gofs, rbp, z, active, shift, msg, ab, a, b);
     }
 }
 
@@ -727,6 +738,9 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
     T* post = post_all + (lane_ok ? g : 0) * n;
     const float* llr = a.llr + (a.llr_off ? a.llr_off[id] : uint64_t(id) * uint64_t(c.e));
     const int limit = a.max_it ? int(a.max_it[id]) : c.max_iterations;
+    // ZC: the row's byte offset in a column and that minus Z (mod 2^16), in 16-bit halves
+    const uint32_t rb0 = uint32_t(r) * uint32_t(sizeof(T));
+    const uint32_t rbp = ZC ? ((rb0 - uint32_t(ZC) * uint32_t(sizeof(T))) << 16) | rb0 : 0u;
 
     if (c.direct) place_direct<BG, HALF>(post, llr, c, r, z, lane_ok);
     else if (lane_ok) place_llrs<BG, HALF>(post, llr, c, r, z);
@@ -784,7 +798,8 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
         if (!any_lane(sh, !finished, round)) break;
         ++it;
         const float* ab = learned ? a.params + size_t(it - 1) * 2 * Graph<BG>::rows : nullptr;
-        sweep<BG, HALF, SOLO, ZC, NMS, 0>(post_all, r, uint32_t(g * n), z, !finished, sh.shift, msg, ab, c.a, c.b);
+        sweep<BG, HALF, SOLO, ZC, NMS, 0>(post_all, r, uint32_t(g * n), rbp, z, !finished, sh.shift, msg, ab, c.a,
+                                          c.b);
         __syncthreads();
         const bool fails = fails_now();
         if (!finished && !fails) {
