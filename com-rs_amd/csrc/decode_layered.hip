@@ -40,13 +40,12 @@ hipError_t layered_occupancy(int bg_mask, bool half, bool solo, int zc, int thre
 hipError_t layered_prepare(const LayeredLaunch& l) {
     const void* k = select_kernel(l.bg_mask, l.half, l.solo, l.zc, l.nms);
     if (!k || l.lds > kLdsBudget) return hipErrorInvalidValue;
-    if (l.zc) {
-        // these kernels address the posteriors from LDS address 0: no static LDS before them
-        hipFuncAttributes attr{};
-        hipError_t err = hipFuncGetAttributes(&attr, k);
-        if (err != hipSuccess) return err;
-        if (attr.sharedSizeBytes != 0) return hipErrorInvalidDeviceFunction;
-    }
+    // every instantiation addresses the posteriors by LDS byte address from 0: no static LDS
+    // before them
+    hipFuncAttributes attr{};
+    hipError_t err = hipFuncGetAttributes(&attr, k);
+    if (err != hipSuccess) return err;
+    if (attr.sharedSizeBytes != 0) return hipErrorInvalidDeviceFunction;
     return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(l.lds));
 }
 

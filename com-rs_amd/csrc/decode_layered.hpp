@@ -49,6 +49,15 @@ namespace layered {
 // two 16-bit lanes of one register (packed integer arithmetic)
 typedef uint16_t u16x2 __attribute__((ext_vector_type(2)));
 
+// (rb + s) mod Zb for byte offsets rb, s < Zb <= 1536: rbp holds rb and rb - Zb (mod 2^16)
+// in its halves and s2 holds s in both, so one 32-bit add forms t = rb + s and t - Zb at
+// once (t < 3072: no carry from the low half; the high half's carry leaves the word), and
+// their unsigned 16-bit min is the one in [0, Zb)
+__device__ __forceinline__ uint32_t rotate_bytes(uint32_t rbp, uint32_t s2) {
+    const u16x2 t = __builtin_bit_cast(u16x2, rbp + s2);
+    return min(uint32_t(t.x), uint32_t(t.y));
+}
+
 template <int BG>
 struct Graph;
 template <>
@@ -168,6 +177,14 @@ struct Store<true> {
     }
 };
 
+// rbp of row r for lifting size z: r and r - z in bytes of the storage, in 16-bit halves
+template <bool HALF>
+__device__ __forceinline__ uint32_t row_pair(int r, int z) {
+    constexpr uint32_t es = sizeof(typename Store<HALF>::T);
+    const uint32_t rb = uint32_t(r) * es;
+    return ((rb - uint32_t(z) * es) << 16) | rb;
+}
+
 // The messages of one lane's rows: output magnitudes g(m1), g(m2) and, packed, the
 // output sign bits and the index of the edge that takes g(m2) (bits 24..28).
 template <int ROWS, bool HALF>
@@ -248,77 +265,60 @@ struct Messages<ROWS, true> {
 // funnel shift) and one index whose magnitude is g(m2); a message is rebuilt as that
 // magnitude with its sign bit replaced by the stored one, which is the output bit for bit.
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, int I>
-__device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post, int r, uint32_t gofs, uint32_t rbp,
-                                      int z, const uint32_t* __restrict__ shift,
+__device__ __forceinline__ void layer(uint32_t gofs, uint32_t rbp, const uint2* __restrict__ edge,
                                       Messages<Graph<BG>::rows, HALF>& msg, float a, float b) {
     using S = Store<HALF>;
     constexpr int e0 = row_begin<BG>(I);
     constexpr int deg = row_begin<BG>(I + 1) - e0;
-    // Z and the codeword's first posterior, opaque to the compiler: it would otherwise keep
-    // one base address per column (68 of them) in registers through the whole iteration.
-    // With Z known at compile time (ZC) the column bases are immediate offsets.
+    static_assert(ZC == 0 || SOLO, "a compile-time Z serves tasks of one codeword");
+    // Addresses are LDS byte addresses (the posteriors start at address 0). rbp: the row's
+    // byte offset in a column and that minus Z, in 16-bit halves; gofs: the byte offset of
+    // the lane's codeword. With Z known at compile time (ZC) the shifts and the column bases
+    // are constants too, so rbp is made opaque, or the compiler would compute all 316
+    // loop-invariant addresses once and hold them in registers; the column bases are then
+    // the LDS instructions' immediate offsets.
     uint32_t base = 0;
-    constexpr uint32_t es0 = sizeof(typename S::T);
-    uint32_t rb = uint32_t(r) * es0;  // the row's byte offset in a column (ZC)
-    if constexpr (ZC != 0) {
-        // the shifts are constants too, so the row offset is what keeps the addresses (all
-        // 316 of them loop-invariant) from being computed once and held in registers
-        z = ZC;
-        asm volatile("" : "+v"(rb), "+v"(rbp));
-    } else {
-        asm volatile("" : "+s"(z));
-    }
+    if constexpr (ZC != 0) asm volatile("" : "+v"(rbp));
     if constexpr (!SOLO) {
         base = gofs;
         asm volatile("" : "+v"(base));
     }
+    const uint32_t rb = rbp & 0xffffu;
     float g1o, g2o;
     msg.template get<I>(g1o, g2o);
     const uint32_t pko = msg.pk[I];
     const uint32_t i1o = pko >> 24;
     constexpr uint32_t es = sizeof(typename S::T);
-    // (r + shift) mod z of entry e: the unsigned min of t and t - z (which wraps when
-    // t < z); with ZC, the LDS byte address (the posteriors start at 0) so that the column
-    // base is the LDS instruction's immediate offset, and a zero shift (every extension
-    // column) needs no wrap; otherwise the element index
-    // With Z known only at run time the layer's shifts come from the LDS table, all of them
-    // before any posterior (the empty asm keeps the compiler from interleaving the two kinds
-    // of reads, which would cost an LDS round trip per pair of edges).
-    uint32_t sq[ZC != 0 ? 1 : deg];
+    // The posterior of entry e on the lane's row: (r + shift) mod Z in column j. With ZC a
+    // zero shift (every extension column) needs no wrap. With Z known only at run time the
+    // layer's entries come from the LDS table, all of them before any posterior (the empty
+    // asm keeps the compiler from interleaving the two kinds of reads, which would cost an
+    // LDS round trip per pair of edges).
+    uint2 ed[ZC != 0 ? 1 : deg];
     if constexpr (ZC == 0) {
 #pragma unroll
-        for (int q = 0; q < deg; ++q) sq[q] = shift[e0 + q];
+        for (int q = 0; q < deg; ++q) ed[q] = edge[e0 + q];
         asm volatile("" ::: "memory");
     }
-    auto address = [&](int q) -> int {
+    auto address = [&](int q) -> uint32_t {
         const int e = e0 + q;
         if constexpr (ZC != 0) {
             const uint32_t s_q = shift_of<BG, ZC>(e) * es;
-            uint32_t rr = rb;
-            if (s_q != 0) {
-                // rbp holds rb and rb - Z (mod 2^16) in its halves, so one 32-bit add of s to
-                // both halves gives t = rb + s and t - Z at once (t < 2 Z * 4 <= 3072: no carry
-                // from the low half; the high half's carry leaves the word), and their
-                // unsigned 16-bit min is t mod Z
-                const u16x2 t = __builtin_bit_cast(u16x2, rbp + (s_q | (s_q << 16)));
-                rr = min(uint32_t(t.x), uint32_t(t.y));
-            }
-            return int(rr) + int(Graph<BG>::t[e].j) * ZC * int(es);
+            const uint32_t rr = s_q == 0 ? rb : rotate_bytes(rbp, s_q | (s_q << 16));
+            return rr + uint32_t(Graph<BG>::t[e].j) * uint32_t(ZC) * es;
         } else {
-            const uint32_t t = uint32_t(r) + sq[ZC != 0 ? 0 : q];
-            const uint32_t rr = min(t, t - uint32_t(z));
-            return int(Graph<BG>::t[e].j) * z + int(rr) + int(base);
+            return rotate_bytes(rbp, ed[ZC != 0 ? 0 : q].x) + ed[ZC != 0 ? 0 : q].y + base;
         }
     };
-    auto load = [&](int ad) -> float { return ZC != 0 ? S::ld_at(uint32_t(ad)) : S::ld(post, ad); };
+    auto load = [&](uint32_t ad) -> float { return S::ld_at(ad); };
     // FP16 kernels run three waves a SIMD (168 registers): their widest layers compute the
     // addresses twice rather than hold them
     constexpr bool keep = !HALF || deg <= 10;
     float v2c[deg];
-    int addr[keep ? deg : 1];
+    uint32_t addr[keep ? deg : 1];
 #pragma unroll
     for (int q = 0; q < deg; ++q) {
-        const int ad = address(q);
+        const uint32_t ad = address(q);
         if constexpr (keep) addr[q] = ad;
         const float mag = uint32_t(q) == i1o ? g2o : g1o;
         // bit 31 from the stored word, the rest from the magnitude: (c ? a : b) bitwise
@@ -374,18 +374,15 @@ __device__ __forceinline__ void layer(typename Store<HALF>::T* __restrict__ post
         const uint32_t ob = __builtin_amdgcn_bitop3_b32(__float_as_uint(v2c[q]), sel, 0x80000000u, 0x6c);
         signs = __builtin_amdgcn_alignbit(signs, ob, 31u);
         const float out = __uint_as_float(ob);
-        const int ad = keep ? addr[keep ? q : 0] : address(q);
-        if constexpr (ZC != 0) S::st_at(uint32_t(ad), v2c[q] + out);
-        else S::st(post, ad, v2c[q] + out);
+        const uint32_t ad = keep ? addr[keep ? q : 0] : address(q);
+        S::st_at(ad, v2c[q] + out);
     }
     msg.pk[I] = signs | (i1 << 24);
 }
 
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, int I>
-__device__ __forceinline__ void sweep(typename Store<HALF>::T* post, int r, uint32_t gofs, uint32_t rbp, int z,
-                                      bool active,
-                                      const uint32_t* shift, Messages<Graph<BG>::rows, HALF>& msg, const float* ab,
-                                      float a, float b) {
+__device__ __forceinline__ void sweep(uint32_t gofs, uint32_t rbp, bool active, const uint2* edge,
+                                      Messages<Graph<BG>::rows, HALF>& msg, const float* ab, float a, float b) {
     if constexpr (I < Graph<BG>::rows) {
         if (!NMS && ab) {
             a = ab[2 * I];
@@ -394,8 +391,8 @@ __device__ __forceinline__ void sweep(typename Store<HALF>::T* post, int r, uint
         if constexpr (Sched<BG>::s.barrier[I]) __syncthreads();
         // (the branch also bounds the scheduler's reach: without it the compiler interleaves
         // layers and spills hundreds of registers)
-        if (active) layer<BG, HALF, SOLO, ZC, NMS, I>(post, r, gofs, rbp, z, shift, msg, a, b);
-        sweep<BG, HALF, SOLO, ZC, NMS, I + 1>(post, r, gofs, rbp, z, active, shift, msg, ab, a, b);
+        if (active) layer<BG, HALF, SOLO, ZC, NMS, I>(gofs, rbp, edge, msg, a, b);
+        sweep<BG, HALF, SOLO, ZC, NMS, I + 1>(gofs, rbp, active, edge, msg, ab, a, b);
     }
 }
 
@@ -411,8 +408,9 @@ constexpr int kSyndSlots = Graph<1>::rows * max_degree<1>() > Graph<2>::rows * m
 
 // The tables of the task's class in LDS, and the workgroup's scratch.
 struct Shared {
-    uint32_t shift[NR_BG1_ENTRIES];
-    uint16_t col[NR_BG1_ENTRIES];
+    // per base entry: its shift in bytes in both 16-bit halves, and its column's first
+    // posterior (the byte offset of column j, j Z times the element size)
+    uint2 edge[NR_BG1_ENTRIES];
     uint16_t row_start[48];
     // Z % 32 == 0: per row, max_degree entries (word of the edge's column, bit) in the packed
     // decisions, the row's edges then entries of the zero column
@@ -520,13 +518,13 @@ __device__ __forceinline__ bool syndrome_fails(const typename Store<HALF>::T* po
         }
     } else if (lane_ok) {
         // every lane checks its rows edge by edge
-        const typename S::T* post = post_all + g * n;
+        const uint32_t rbp = row_pair<HALF>(r, z), gb = uint32_t(g * n) * uint32_t(sizeof(typename S::T));
         uint32_t bad = 0;
         for (int i = 0; i < Graph<BG>::rows && !bad; ++i) {
             uint32_t par = 0;
             for (int eb = sh.row_start[i]; eb < sh.row_start[i + 1]; ++eb) {
-                const uint32_t t = uint32_t(r) + sh.shift[eb];
-                par ^= S::sign(post, sh.col[eb] * z + int(min(t, t - uint32_t(z))));
+                const uint2 ed = sh.edge[eb];
+                par ^= S::sign_at(rotate_bytes(rbp, ed.x) + ed.y + gb);
             }
             bad |= par;
         }
@@ -554,8 +552,7 @@ struct Precheck<2> {
 };
 
 template <int BG, bool HALF, int ZC, int L>
-__device__ __forceinline__ uint32_t row_parity(const typename Store<HALF>::T* post, int r, int z,
-                                               const uint32_t* shift) {
+__device__ __forceinline__ uint32_t row_parity(uint32_t rbp, uint32_t gbytes, const uint2* edge) {
     using S = Store<HALF>;
     constexpr int e0 = row_begin<BG>(L), deg = row_begin<BG>(L + 1) - e0;
     constexpr uint32_t es = sizeof(typename S::T);
@@ -566,12 +563,11 @@ __device__ __forceinline__ uint32_t row_parity(const typename Store<HALF>::T* po
         const int e = e0 + q;
         if constexpr (ZC != 0) {
             const uint32_t s_q = shift_of<BG, ZC>(e) * es;
-            const uint32_t t = uint32_t(r) * es + s_q;
-            const uint32_t rr = s_q == 0 ? t : min(t, t - uint32_t(ZC) * es);
+            const uint32_t rr = s_q == 0 ? (rbp & 0xffffu) : rotate_bytes(rbp, s_q | (s_q << 16));
             sg[q] = S::sign_at(rr + uint32_t(Graph<BG>::t[e].j) * ZC * es);
         } else {
-            const uint32_t t = uint32_t(r) + shift[e];
-            sg[q] = S::sign(post, int(Graph<BG>::t[e].j) * z + int(min(t, t - uint32_t(z))));
+            const uint2 ed = edge[e];
+            sg[q] = S::sign_at(rotate_bytes(rbp, ed.x) + ed.y + gbytes);
         }
     }
     uint32_t par = 0;
@@ -586,8 +582,9 @@ template <int BG, bool HALF, int ZC>
 __device__ __forceinline__ bool solo_syndrome_fails(const typename Store<HALF>::T* post_all, uint32_t* hd_all,
                                                     Shared& sh, uint32_t stamp, uint32_t& round, int z, int n,
                                                     int r, bool lane_ok) {
-    const uint32_t bad = lane_ok ? row_parity<BG, HALF, ZC, Precheck<BG>::layers[0]>(post_all, r, z, sh.shift) |
-                                       row_parity<BG, HALF, ZC, Precheck<BG>::layers[1]>(post_all, r, z, sh.shift)
+    const uint32_t rbp = row_pair<HALF>(r, z);
+    const uint32_t bad = lane_ok ? row_parity<BG, HALF, ZC, Precheck<BG>::layers[0]>(rbp, 0u, sh.edge) |
+                                       row_parity<BG, HALF, ZC, Precheck<BG>::layers[1]>(rbp, 0u, sh.edge)
                                  : 0u;
     if (any_lane(sh, bad != 0u, round)) return true;
     syndrome_fails<BG, HALF>(post_all, hd_all, sh, stamp, z, n, 1, 0, r, lane_ok);
@@ -601,9 +598,9 @@ template <int BG, bool HALF, int ZC>
 __device__ __forceinline__ bool group_syndrome_fails(const typename Store<HALF>::T* post_all, uint32_t* hd_all,
                                                      Shared& sh, uint32_t stamp, uint32_t& round, int z, int n,
                                                      int cnt, int g, int r, bool lane_ok, bool finished) {
-    const typename Store<HALF>::T* post = post_all + g * n;
-    const uint32_t bad = lane_ok ? row_parity<BG, HALF, ZC, Precheck<BG>::layers[0]>(post, r, z, sh.shift) |
-                                       row_parity<BG, HALF, ZC, Precheck<BG>::layers[1]>(post, r, z, sh.shift)
+    const uint32_t rbp = row_pair<HALF>(r, z), gb = uint32_t(g * n) * uint32_t(sizeof(typename Store<HALF>::T));
+    const uint32_t bad = lane_ok ? row_parity<BG, HALF, ZC, Precheck<BG>::layers[0]>(rbp, gb, sh.edge) |
+                                       row_parity<BG, HALF, ZC, Precheck<BG>::layers[1]>(rbp, gb, sh.edge)
                                  : 0u;
     if (bad) sh.fails[g] = stamp;
     __syncthreads();
@@ -738,9 +735,10 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
     T* post = post_all + (lane_ok ? g : 0) * n;
     const float* llr = a.llr + (a.llr_off ? a.llr_off[id] : uint64_t(id) * uint64_t(c.e));
     const int limit = a.max_it ? int(a.max_it[id]) : c.max_iterations;
-    // ZC: the row's byte offset in a column and that minus Z (mod 2^16), in 16-bit halves
-    const uint32_t rb0 = uint32_t(r) * uint32_t(sizeof(T));
-    const uint32_t rbp = ZC ? ((rb0 - uint32_t(ZC) * uint32_t(sizeof(T))) << 16) | rb0 : 0u;
+    // the row's byte offset in a column and that minus Z (mod 2^16), in 16-bit halves, and
+    // the byte offset of the lane's codeword (the posteriors start at LDS address 0)
+    const uint32_t rbp = row_pair<HALF>(r, z);
+    const uint32_t gbytes = uint32_t(lane_ok ? g : 0) * uint32_t(n) * uint32_t(sizeof(T));
 
     if (c.direct) place_direct<BG, HALF>(post, llr, c, r, z, lane_ok);
     else if (lane_ok) place_llrs<BG, HALF>(post, llr, c, r, z);
@@ -798,8 +796,7 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
         if (!any_lane(sh, !finished, round)) break;
         ++it;
         const float* ab = learned ? a.params + size_t(it - 1) * 2 * Graph<BG>::rows : nullptr;
-        sweep<BG, HALF, SOLO, ZC, NMS, 0>(post_all, r, uint32_t(g * n), rbp, z, !finished, sh.shift, msg, ab, c.a,
-                                          c.b);
+        sweep<BG, HALF, SOLO, ZC, NMS, 0>(gbytes, rbp, !finished, sh.edge, msg, ab, c.a, c.b);
         __syncthreads();
         const bool fails = fails_now();
         if (!finished && !fails) {
@@ -904,8 +901,8 @@ __global__ void __launch_bounds__(kMaxThreads, HALF ? 3 : 1) decode_layered_kern
             const int maxd = c.bg == 1 ? maxd1 : maxd2;
             const uint32_t w32 = uint32_t(c.z >> 5);
             for (int e = threadIdx.x; e < edges; e += blockDim.x) {
-                sh.shift[e] = c.shift32[e];
-                sh.col[e] = uint16_t(c.col[e]);
+                const uint32_t sb = c.shift32[e] * uint32_t(HALF ? 2 : 4);
+                sh.edge[e] = uint2{sb | (sb << 16), uint32_t(c.col[e]) * uint32_t(c.z) * uint32_t(HALF ? 2 : 4)};
             }
             for (int i = threadIdx.x; i <= rows; i += blockDim.x) sh.row_start[i] = c.row_start[i];
             // the funnel shifts of the packed syndrome, row by row: edge (j, v) reads word
