@@ -86,12 +86,16 @@ void run_layered(const cf_code* h, int device, const LayeredRun& r, hipStream_t 
     const int zc = c.z == 384 && solo ? 384 : 0;
     // the normalized min-sum's own instantiation (no offset, no learned parameters)
     const bool nms = zc == 384 && !r.params && st.rule == 1;
+    // FP32 with Z = 384: two sets of six waves a codeword, the layers of a barrier group side
+    // by side (twelve waves a CU, three a SIMD, where one set alone sits 2:2:1:1)
+    const bool dual = zc == 384 && !r.half;
+    const int lanes = dual ? 2 * kMaxThreads : threads;
     const size_t lds = off + kSharedBytes;
     int per_cu = 0, cus = 0;
-    check_hip(layered_occupancy(c.bg, r.half, solo, zc, threads, lds, &per_cu, nms), "occupancy");
+    check_hip(layered_occupancy(c.bg, r.half, solo, zc, lanes, lds, &per_cu, nms, dual), "occupancy");
     check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "device attribute");
     const int grid = int(std::min<int64_t>(a.n_tasks, int64_t(std::max(per_cu, 1)) * std::max(cus, 1)));
-    LayeredLaunch l{c.bg, r.half, solo, zc, threads, std::max(grid, 1), lds, nms};
+    LayeredLaunch l{c.bg, r.half, solo, zc, lanes, std::max(grid, 1), lds, nms, dual};
     if (counter) {
         // the caller's counter: a fill kernel (no memset node) zeroes it before the launch
         check_hip(launch_zero(reinterpret_cast<uint64_t*>(counter), 1, stream), "zero launch");

@@ -6,14 +6,15 @@
 
 namespace comfec {
 
-const void* layered_kernel_f32(int bg_mask, bool solo, int zc, bool nms);
+const void* layered_kernel_f32(int bg_mask, bool solo, int zc, bool nms, bool dual);
 const void* layered_kernel_f16(int bg_mask, bool solo, int zc, bool nms);
 
 namespace {
 
-const void* select_kernel(int bg_mask, bool half, bool solo, int zc, bool nms) {
+const void* select_kernel(int bg_mask, bool half, bool solo, int zc, bool nms, bool dual) {
     if (bg_mask < 1 || bg_mask > 3) return nullptr;
-    return half ? layered_kernel_f16(bg_mask, solo, zc, nms) : layered_kernel_f32(bg_mask, solo, zc, nms);
+    if (dual && (half || !solo || zc != 384 || bg_mask == 3)) return nullptr;
+    return half ? layered_kernel_f16(bg_mask, solo, zc, nms) : layered_kernel_f32(bg_mask, solo, zc, nms, dual);
 }
 
 }  // namespace
@@ -29,8 +30,8 @@ size_t layered_lds_bytes(int bg, int z, int g, bool half) {
 }
 
 hipError_t layered_occupancy(int bg_mask, bool half, bool solo, int zc, int threads, size_t lds,
-                             int* blocks_per_cu, bool nms) {
-    const void* k = select_kernel(bg_mask, half, solo, zc, nms);
+                             int* blocks_per_cu, bool nms, bool dual) {
+    const void* k = select_kernel(bg_mask, half, solo, zc, nms, dual);
     if (!k) return hipErrorInvalidValue;
     hipError_t err = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (err != hipSuccess) return err;
@@ -38,7 +39,7 @@ hipError_t layered_occupancy(int bg_mask, bool half, bool solo, int zc, int thre
 }
 
 hipError_t layered_prepare(const LayeredLaunch& l) {
-    const void* k = select_kernel(l.bg_mask, l.half, l.solo, l.zc, l.nms);
+    const void* k = select_kernel(l.bg_mask, l.half, l.solo, l.zc, l.nms, l.dual);
     if (!k || l.lds > kLdsBudget) return hipErrorInvalidValue;
     // every instantiation addresses the posteriors by LDS byte address from 0: no static LDS
     // before them
@@ -50,8 +51,10 @@ hipError_t layered_prepare(const LayeredLaunch& l) {
 }
 
 hipError_t launch_layered(const LayeredArgs& a, const LayeredLaunch& l, hipStream_t stream, bool prepare) {
-    const void* k = select_kernel(l.bg_mask, l.half, l.solo, l.zc, l.nms);
-    if (!k || l.threads < 64 || l.threads > kMaxThreads || l.threads % 64 != 0 || l.grid < 1)
+    const void* k = select_kernel(l.bg_mask, l.half, l.solo, l.zc, l.nms, l.dual);
+    // two wave sets take exactly 2 x 384 lanes
+    if (!k || l.grid < 1 || (l.dual ? l.threads != 2 * kMaxThreads
+                                    : (l.threads < 64 || l.threads > kMaxThreads || l.threads % 64 != 0)))
         return hipErrorInvalidValue;
     if (l.lds > kLdsBudget) return hipErrorInvalidValue;
     if (prepare) {

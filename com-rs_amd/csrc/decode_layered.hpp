@@ -108,6 +108,71 @@ struct Sched {
     static constexpr Schedule<BG> s = make_schedule<BG>();
 };
 
+// Two sets of six waves on one codeword of Z = 384 (the DUAL instantiations, 768 lanes): a
+// CU places wave i and waves i + 4, i + 8 on one SIMD, so with set s = waves 6s .. 6s + 5
+// every SIMD holds waves of both sets. The layers between two barriers share no column, so
+// the layers of such a group are spread over the sets (the widest first, each to the set
+// with fewer edges of the group so far, ties to the set with fewer layers) and run side by
+// side; a group of one layer runs on one set while the other waits at the next barrier.
+// A lane keeps the messages of its set's layers only: slot[l] is layer l's place among
+// them, and slots the larger count.
+template <int BG>
+struct DualPlan {
+    int owner[Graph<BG>::rows];
+    int slot[Graph<BG>::rows];
+    int slots;
+};
+
+template <int BG>
+constexpr int degree_of(int i) {
+    return row_begin<BG>(i + 1) - row_begin<BG>(i);
+}
+
+template <int BG>
+constexpr DualPlan<BG> make_dual() {
+    constexpr int R = Graph<BG>::rows;
+    DualPlan<BG> d{};
+    int count[2] = {0, 0};
+    for (int l = 0; l < R;) {
+        int end = l + 1;
+        while (end < R && !Sched<BG>::s.barrier[end]) ++end;
+        int load[2] = {0, 0};
+        bool placed[R] = {};
+        for (int n = l; n < end; ++n) {
+            int best = -1;
+            for (int k = l; k < end; ++k)
+                if (!placed[k] && (best < 0 || degree_of<BG>(k) > degree_of<BG>(best))) best = k;
+            const int set = load[0] != load[1] ? (load[0] < load[1] ? 0 : 1) : (count[0] <= count[1] ? 0 : 1);
+            placed[best] = true;
+            d.owner[best] = set;
+            load[set] += degree_of<BG>(best);
+            ++count[set];
+        }
+        l = end;
+    }
+    int next[2] = {0, 0};
+    for (int i = 0; i < R; ++i) d.slot[i] = next[d.owner[i]]++;
+    d.slots = count[0] > count[1] ? count[0] : count[1];
+    return d;
+}
+
+template <int BG>
+struct Dual {
+    static constexpr DualPlan<BG> p = make_dual<BG>();
+};
+
+// the message slots a lane holds
+template <int BG, bool DUAL>
+constexpr int msg_slots() {
+    if constexpr (DUAL) return Dual<BG>::p.slots;
+    else return Graph<BG>::rows;
+}
+template <int BG, bool DUAL>
+constexpr int slot_of(int i) {
+    if constexpr (DUAL) return Dual<BG>::p.slot[i];
+    else return i;
+}
+
 // The lifting set of Z and the shift V mod Z of base entry e, when Z is known at compile
 // time (nr/ldpc.rs:117-133).
 constexpr int lifting_set_of(int z) {
@@ -264,10 +329,11 @@ struct Messages<ROWS, true> {
 // The stored state keeps the output sign bits (edge q at bit deg-1-q, shifted in with a
 // funnel shift) and one index whose magnitude is g(m2); a message is rebuilt as that
 // magnitude with its sign bit replaced by the stored one, which is the output bit for bit.
-template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, int I>
+template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I>
 __device__ __forceinline__ void layer(uint32_t gofs, uint32_t rbp, const uint2* __restrict__ edge,
-                                      Messages<Graph<BG>::rows, HALF>& msg, float a, float b) {
+                                      Messages<msg_slots<BG, DUAL>(), HALF>& msg, float a, float b) {
     using S = Store<HALF>;
+    constexpr int SL = slot_of<BG, DUAL>(I);  // where the lane keeps this layer's messages
     constexpr int e0 = row_begin<BG>(I);
     constexpr int deg = row_begin<BG>(I + 1) - e0;
     static_assert(ZC == 0 || SOLO, "a compile-time Z serves tasks of one codeword");
@@ -285,8 +351,8 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t rbp, const uint2* 
     }
     const uint32_t rb = rbp & 0xffffu;
     float g1o, g2o;
-    msg.template get<I>(g1o, g2o);
-    const uint32_t pko = msg.pk[I];
+    msg.template get<SL>(g1o, g2o);
+    const uint32_t pko = msg.pk[SL];
     const uint32_t i1o = pko >> 24;
     constexpr uint32_t es = sizeof(typename S::T);
     // The posterior of entry e on the lane's row: (r + shift) mod Z in column j. With ZC a
@@ -357,7 +423,7 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t rbp, const uint2* 
     // NMS: b = 0, and a max(m - 0, 0) is a m bit for bit for every magnitude m >= +0
     float g1 = NMS ? a * m1 : a * fmaxf(m1 - b, 0.0f);
     float g2 = NMS ? a * m2 : a * fmaxf(m2 - b, 0.0f);
-    msg.template put<I>(g1, g2);
+    msg.template put<SL>(g1, g2);
     // the magnitudes with the row's sign product applied once: an edge's output is then
     // its choice XOR its own input's sign bit, and the stored bit is the output's sign
     parity &= 0x80000000u;
@@ -377,12 +443,13 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t rbp, const uint2* 
         const uint32_t ad = keep ? addr[keep ? q : 0] : address(q);
         S::st_at(ad, v2c[q] + out);
     }
-    msg.pk[I] = signs | (i1 << 24);
+    msg.pk[SL] = signs | (i1 << 24);
 }
 
-template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, int I>
-__device__ __forceinline__ void sweep(uint32_t gofs, uint32_t rbp, bool active, const uint2* edge,
-                                      Messages<Graph<BG>::rows, HALF>& msg, const float* ab, float a, float b) {
+template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I>
+__device__ __forceinline__ void sweep(uint32_t gofs, uint32_t rbp, bool active, int set, const uint2* edge,
+                                      Messages<msg_slots<BG, DUAL>(), HALF>& msg, const float* ab, float a,
+                                      float b) {
     if constexpr (I < Graph<BG>::rows) {
         if (!NMS && ab) {
             a = ab[2 * I];
@@ -391,8 +458,10 @@ __device__ __forceinline__ void sweep(uint32_t gofs, uint32_t rbp, bool active, 
         if constexpr (Sched<BG>::s.barrier[I]) __syncthreads();
         // (the branch also bounds the scheduler's reach: without it the compiler interleaves
         // layers and spills hundreds of registers)
-        if (active) layer<BG, HALF, SOLO, ZC, NMS, I>(gofs, rbp, edge, msg, a, b);
-        sweep<BG, HALF, SOLO, ZC, NMS, I + 1>(gofs, rbp, active, edge, msg, ab, a, b);
+        // DUAL: set is uniform over a wavefront, so a wave of the other set skips the layer
+        if ((!DUAL || set == Dual<BG>::p.owner[I]) && active)
+            layer<BG, HALF, SOLO, ZC, NMS, DUAL, I>(gofs, rbp, edge, msg, a, b);
+        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, I + 1>(gofs, rbp, active, set, edge, msg, ab, a, b);
     }
 }
 
@@ -718,8 +787,9 @@ __device__ __forceinline__ void place_direct(typename Store<HALF>::T* post, cons
 
 // One task: codewords order[first .. first + cnt) of class c (base graph BG). SOLO: the
 // task has one codeword (G = 1), whose posteriors start the LDS; ZC: Z, when every class of
-// the launch has the same (0: read at run time).
-template <int BG, bool HALF, bool SOLO, int ZC, bool NMS>
+// the launch has the same (0: read at run time); DUAL: two sets of Z = 384 lanes on the
+// codeword, each running its layers (the second set places no LLRs and writes no outputs).
+template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL>
 __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass& c, uint32_t first, int cnt, Shared& sh,
                          void* lds_dyn, uint32_t& stamp, uint32_t& round) {
     using S = Store<HALF>;
@@ -727,10 +797,14 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
     const int z = ZC ? ZC : c.z, n = ZC ? Graph<BG>::cols * ZC : c.n;
     T* post_all = static_cast<T*>(lds_dyn);
     uint32_t* hd_all = reinterpret_cast<uint32_t*>(post_all + c.g * n);  // g n is even: 4-byte aligned
+    static_assert(!DUAL || (SOLO && ZC == 384 && !HALF), "two wave sets serve FP32 tasks of one codeword, Z = 384");
     const int tid = threadIdx.x;
-    const int g = SOLO ? 0 : tid / z, r = tid - g * z;
+    // DUAL: the lane's set, uniform over a wavefront (Z = 384 is six of them)
+    const int set = DUAL ? int(__builtin_amdgcn_readfirstlane(uint32_t(tid >= ZC))) : 0;
+    const int g = SOLO ? 0 : tid / z, r = tid - g * z - set * z;
     // lanes beyond the task's codewords (SOLO: beyond Z) idle
-    const bool lane_ok = SOLO ? tid < z : g < cnt;
+    const bool lane_ok = SOLO ? (DUAL || tid < z) : g < cnt;
+    const bool places = !DUAL || set == 0;  // places the LLRs and writes the outputs
     const uint32_t id = lane_ok ? (a.order ? a.order[first + g] : first + uint32_t(g)) : 0u;
     T* post = post_all + (lane_ok ? g : 0) * n;
     const float* llr = a.llr + (a.llr_off ? a.llr_off[id] : uint64_t(id) * uint64_t(c.e));
@@ -740,8 +814,8 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
     const uint32_t rbp = row_pair<HALF>(r, z);
     const uint32_t gbytes = uint32_t(lane_ok ? g : 0) * uint32_t(n) * uint32_t(sizeof(T));
 
-    if (c.direct) place_direct<BG, HALF>(post, llr, c, r, z, lane_ok);
-    else if (lane_ok) place_llrs<BG, HALF>(post, llr, c, r, z);
+    if (c.direct) place_direct<BG, HALF>(post, llr, c, r, z, lane_ok && places);
+    else if (lane_ok && places) place_llrs<BG, HALF>(post, llr, c, r, z);
     if ((z & 31) == 0) {
         // the zero column of every codeword's packed decisions
         const int w32 = z >> 5, per_cw = (Graph<BG>::cols + 1) * 2 * w32;
@@ -752,7 +826,7 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
     }
     __syncthreads();
 
-    Messages<Graph<BG>::rows, HALF> msg;
+    Messages<msg_slots<BG, DUAL>(), HALF> msg;
     msg.zero();
     bool finished = !lane_ok, restarted = false;
     uint32_t its = 0, ok = 0;
@@ -778,7 +852,7 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
             if (!finished) {
                 if (a.fallback) {
                     restarted = true;
-                    place_llrs<BG, HALF>(post, llr, c, r, z);
+                    if (places) place_llrs<BG, HALF>(post, llr, c, r, z);
                     msg.zero();
                 } else {
                     finished = true;
@@ -796,7 +870,7 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
         if (!any_lane(sh, !finished, round)) break;
         ++it;
         const float* ab = learned ? a.params + size_t(it - 1) * 2 * Graph<BG>::rows : nullptr;
-        sweep<BG, HALF, SOLO, ZC, NMS, 0>(gbytes, rbp, !finished, sh.edge, msg, ab, c.a, c.b);
+        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, 0>(gbytes, rbp, !finished, set, sh.edge, msg, ab, c.a, c.b);
         __syncthreads();
         const bool fails = fails_now();
         if (!finished && !fails) {
@@ -806,7 +880,7 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
         }
     }
 
-    if (lane_ok) {
+    if (lane_ok && places) {
         if (r == 0) {
             if (a.iterations) a.iterations[id] = its;
             if (a.ok) a.ok[id] = uint8_t(ok);
@@ -856,9 +930,9 @@ __device__ __forceinline__ BatchClass uni(const BatchClass& c) {
 
 // Workgroups take tasks until none is left. BGMASK: the base graphs of the classes; SOLO:
 // every class has G = 1; ZC: every class has Z = ZC (0: any); NMS: every class runs the
-// normalized min-sum and no learned parameters are given.
-template <int BGMASK, bool HALF, bool SOLO, int ZC, bool NMS = false>
-__global__ void __launch_bounds__(kMaxThreads, HALF ? 3 : 1) decode_layered_kernel(LayeredArgs a) {
+// normalized min-sum and no learned parameters are given; DUAL: two wave sets a codeword.
+template <int BGMASK, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL>
+__device__ __forceinline__ void layered_body(LayeredArgs a) {
     // one dynamic allocation and no static LDS, so that the posteriors start at address 0
     // and the column bases are immediate offsets of the LDS instructions
     extern __shared__ __align__(16) uint32_t lds_dyn[];
@@ -920,13 +994,24 @@ __global__ void __launch_bounds__(kMaxThreads, HALF ? 3 : 1) decode_layered_kern
             loaded = cls;
         }
         if constexpr ((BGMASK & 1) != 0) {
-            if (c.bg == 1) run_task<1, HALF, SOLO, ZC, NMS>(a, c, first, cnt, sh, lds_dyn, stamp, round);
+            if (c.bg == 1) run_task<1, HALF, SOLO, ZC, NMS, DUAL>(a, c, first, cnt, sh, lds_dyn, stamp, round);
         }
         if constexpr ((BGMASK & 2) != 0) {
-            if (c.bg == 2) run_task<2, HALF, SOLO, ZC, NMS>(a, c, first, cnt, sh, lds_dyn, stamp, round);
+            if (c.bg == 2) run_task<2, HALF, SOLO, ZC, NMS, DUAL>(a, c, first, cnt, sh, lds_dyn, stamp, round);
         }
         if (!a.queue) t += gridDim.x;
     }
+}
+
+template <int BGMASK, bool HALF, bool SOLO, int ZC, bool NMS = false>
+__global__ void __launch_bounds__(kMaxThreads, HALF ? 3 : 1) decode_layered_kernel(LayeredArgs a) {
+    layered_body<BGMASK, HALF, SOLO, ZC, NMS, false>(a);
+}
+
+// 768 lanes: twelve waves a CU, three on every SIMD (at most 168 registers a lane)
+template <int BG, bool NMS>
+__global__ void __launch_bounds__(2 * kMaxThreads, 1) decode_layered_dual_kernel(LayeredArgs a) {
+    layered_body<BG, false, true, 384, NMS, true>(a);
 }
 
 }  // namespace layered

@@ -4,8 +4,16 @@
 
 namespace comfec {
 
-const void* layered_kernel_f32(int bg_mask, bool solo, int zc, bool nms) {
+const void* layered_kernel_f32(int bg_mask, bool solo, int zc, bool nms, bool dual) {
     using namespace layered;
+    if (dual) {
+        if (!solo || zc != 384) return nullptr;
+        if (bg_mask == 1) return nms ? reinterpret_cast<const void*>(decode_layered_dual_kernel<1, true>)
+                                     : reinterpret_cast<const void*>(decode_layered_dual_kernel<1, false>);
+        if (bg_mask == 2) return nms ? reinterpret_cast<const void*>(decode_layered_dual_kernel<2, true>)
+                                     : reinterpret_cast<const void*>(decode_layered_dual_kernel<2, false>);
+        return nullptr;
+    }
     if (solo && zc == 384) {
         if (nms) {
             if (bg_mask == 1) return reinterpret_cast<const void*>(decode_layered_kernel<1, false, true, 384, true>);

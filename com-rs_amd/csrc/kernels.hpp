@@ -101,6 +101,7 @@ struct LayeredLaunch {
     int grid;
     size_t lds;                 // dynamic LDS: layered_shared_offset + kSharedBytes
     bool nms = false;           // every class the normalized min-sum and no learned parameters (solo, zc 384)
+    bool dual = false;          // two wave sets a codeword (FP32, solo, zc 384): 2 x 384 lanes
 };
 
 bool layered_supported(int bg, int z);
@@ -109,7 +110,7 @@ size_t layered_lds_bytes(int bg, int z, int g, bool half);
 // where the tables go: after the largest task's LDS, 16-byte aligned
 inline size_t layered_shared_offset(size_t task_lds) { return (task_lds + 15) & ~size_t(15); }
 hipError_t layered_occupancy(int bg_mask, bool half, bool solo, int zc, int threads, size_t lds, int* blocks_per_cu,
-                             bool nms = false);
+                             bool nms = false, bool dual = false);
 // sets the kernel's dynamic LDS limit for the launch (not while a stream is being captured)
 hipError_t layered_prepare(const LayeredLaunch& l);
 hipError_t launch_layered(const LayeredArgs& a, const LayeredLaunch& l, hipStream_t stream, bool prepare);
