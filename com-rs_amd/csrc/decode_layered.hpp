@@ -251,7 +251,9 @@ __device__ __forceinline__ uint32_t row_pair(int r, int z) {
 }
 
 // The messages of one lane's rows: output magnitudes g(m1), g(m2) and, packed, the
-// output sign bits and the index of the edge that takes g(m2) (bits 24..28).
+// index of the edge that takes g(m2) (bits 0..4) and the output sign bits (edge q of a row
+// of degree deg at bit 5 + deg - 1 - q). The index in the low bits is built from inline
+// constants and compared with them, so no register holds an edge number.
 template <int ROWS, bool HALF>
 struct Messages;
 
@@ -326,7 +328,7 @@ struct Messages<ROWS, true> {
 //    m2 = m1); they are selected, never computed, so the order of the comparisons does not
 //    matter: three inputs at a time give their min3 and med3, and a pair (t1 <= t2) merges
 //    into (m1 <= m2) as min(m1, t1) and med3(m1, t1, min(m2, t2)).
-// The stored state keeps the output sign bits (edge q at bit deg-1-q, shifted in with a
+// The stored state keeps the output sign bits (edge q at bit 5 + deg-1-q, shifted in with a
 // funnel shift) and one index whose magnitude is g(m2); a message is rebuilt as that
 // magnitude with its sign bit replaced by the stored one, which is the output bit for bit.
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I>
@@ -353,7 +355,7 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t rbp, const uint2* 
     float g1o, g2o;
     msg.template get<SL>(g1o, g2o);
     const uint32_t pko = msg.pk[SL];
-    const uint32_t i1o = pko >> 24;
+    const uint32_t i1o = pko & 31u;
     constexpr uint32_t es = sizeof(typename S::T);
     // The posterior of entry e on the lane's row: (r + shift) mod Z in column j. With ZC a
     // zero shift (every extension column) needs no wrap. With Z known only at run time the
@@ -389,7 +391,7 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t rbp, const uint2* 
         const float mag = uint32_t(q) == i1o ? g2o : g1o;
         // bit 31 from the stored word, the rest from the magnitude: (c ? a : b) bitwise
         const float old = __uint_as_float(
-            __builtin_amdgcn_bitop3_b32(pko << (32 - deg + q), __float_as_uint(mag), 0x80000000u, 0xe4));
+            __builtin_amdgcn_bitop3_b32(pko << (27 - deg + q), __float_as_uint(mag), 0x80000000u, 0xe4));
         v2c[q] = load(ad) - old;
     }
     // the row's sign product: three-input XORs from the first input on
@@ -443,7 +445,7 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t rbp, const uint2* 
         const uint32_t ad = keep ? addr[keep ? q : 0] : address(q);
         S::st_at(ad, v2c[q] + out);
     }
-    msg.pk[SL] = signs | (i1 << 24);
+    msg.pk[SL] = (signs << 5) | i1;
 }
 
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I>
