@@ -31,10 +31,10 @@ LDS_PUBLISHED_GBS = 256 * 128 * 2.4
 VALU_PUBLISHED_GINST = 1024 * 2.4 / 4
 # What the profiles of the plain launch on this workload measured, per codeword
 # (profiles/decode_layered_bg1_z384_16qam.md): HBM bytes from rocprofv3 FETCH_SIZE (x 2.0)
-# and WRITE_SIZE (x 1.0), the channel LLRs in and the decisions out (453.0 MB per 4096
-# codewords); SQ_INSTS_VALU per codeword and iteration (6 waves)
-TRAFFIC_PER_CODEWORD = 110585
-VALU_PER_CODEWORD_ITERATION = 6 * 5010
+# and WRITE_SIZE (x 1.0), the channel LLRs in and the decisions out (451.8 MB per 4096
+# codewords); SQ_INSTS_VALU per codeword and iteration (both wave sets, 12 waves)
+TRAFFIC_PER_CODEWORD = 110296
+VALU_PER_CODEWORD_ITERATION = 30625
 PROFILE = "profiles/decode_layered_bg1_z384_16qam.md"
 LEARNED_ITERATIONS = 10
 K_INFO, E, Z, BG1_EDGES = 8448, 25344, 384, 316
@@ -305,7 +305,7 @@ def main():
         },
         "roofline": {"bound": "lds", "achieved": round(lds_achieved, 1), "peak": lds_peak, "unit": "GB/s",
                      "frac": round(lds_achieved / lds_peak, 4), "traffic": TRAFFIC_PER_CODEWORD * batch,
-                     "kernel": "decode_layered_kernel<1, false, true, 384, true> (plain step)",
+                     "kernel": "decode_layered_dual_kernel<1, true> (plain step)",
                      "unit_of_work": f"one codeword, I x E_dg x 16 B of LDS (I = {it_plain:.3f}, E_dg = {edges}); "
                                      f"{batch} a launch",
                      "peak_source": "measured: conflict-free 4-byte LDS reads on every CU (peaks.hip)",
