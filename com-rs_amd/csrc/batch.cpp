@@ -53,6 +53,15 @@ int round64(int x) { return (x + 63) / 64 * 64; }
 // codewords per task for lifting size z in a workgroup of `threads` lanes
 int group_of(int z, int threads) { return std::max(1, std::min(threads / z, kMaxGroup)); }
 
+// Whether a launch runs two wave sets a codeword: FP32 tasks of one codeword in workgroups of
+// at least 256 lanes, of which the CU holds one (the two-set kernel holds one too)
+bool two_sets(bool solo, bool half, int threads, int per_cu) { return solo && !half && threads >= 256 && per_cu == 1; }
+
+// the per-code width of the codes that run on two wave sets: one codeword a task, 768 lanes
+constexpr int kDualWidth = 2 * kMaxThreads;
+// codewords per task at a per-code width
+int group_at(int z, int w) { return w == kDualWidth ? 1 : group_of(z, w); }
+
 }  // namespace
 
 // The layered kernel on `count` codewords of one code: a workgroup of the next multiple of
@@ -86,13 +95,15 @@ void run_layered(const cf_code* h, int device, const LayeredRun& r, hipStream_t 
     const int zc = c.z == 384 && solo ? 384 : 0;
     // the normalized min-sum's own instantiation (no offset, no learned parameters)
     const bool nms = zc == 384 && !r.params && st.rule == 1;
-    // FP32 with Z = 384: two sets of six waves a codeword, the layers of a barrier group side
-    // by side (twelve waves a CU, three a SIMD, where one set alone sits 2:2:1:1)
-    const bool dual = zc == 384 && !r.half;
-    const int lanes = dual ? 2 * kMaxThreads : threads;
     const size_t lds = off + kSharedBytes;
     int per_cu = 0, cus = 0;
-    check_hip(layered_occupancy(c.bg, r.half, solo, zc, lanes, lds, &per_cu, nms, dual), "occupancy");
+    check_hip(layered_occupancy(c.bg, r.half, solo, zc, threads, lds, &per_cu, nms), "occupancy");
+    // one FP32 codeword that fills a CU (four to six waves): two sets of six waves on it, the
+    // layers of a barrier group side by side (twelve waves, three a SIMD, where one set alone
+    // sits 2:2:1:1 or 1:1:1:1)
+    const bool dual = two_sets(solo, r.half, threads, per_cu);
+    const int lanes = dual ? 2 * kMaxThreads : threads;
+    if (dual) check_hip(layered_occupancy(c.bg, r.half, solo, zc, lanes, lds, &per_cu, nms, true), "occupancy");
     check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "device attribute");
     const int grid = int(std::min<int64_t>(a.n_tasks, int64_t(std::max(per_cu, 1)) * std::max(cus, 1)));
     LayeredLaunch l{c.bg, r.half, solo, zc, lanes, std::max(grid, 1), lds, nms, dual};
@@ -217,6 +228,16 @@ int modelled_width(const NrCode& c, bool half, uint32_t cap_per_cu) {
         }
     }
     if (!best_w) fail(Kind::Device, "no workgroup width fits the device for Z = " + std::to_string(c.z));
+    // an FP32 codeword that alone fills the CU at that width runs on two wave sets instead
+    const int g = group_of(int(c.z), best_w);
+    if (g == 1 && !half) {
+        const size_t lds = layered_shared_offset(layered_lds_bytes(c.bg, int(c.z), 1, half)) + kSharedBytes;
+        int per_cu = 0;
+        if (layered_occupancy(c.bg, half, true, 0, best_w, lds, &per_cu) == hipSuccess &&
+            two_sets(true, half, best_w, per_cu))
+            return kDualWidth;
+        (void)hipGetLastError();
+    }
     return best_w;
 }
 
@@ -251,7 +272,7 @@ void configure(cf_batch_plan& p, const cf_batch_plan_config& cfg) {
     std::vector<BatchClass> classes(nc);
     for (size_t k = 0; k < nc; ++k) {
         const NrCode& c = p.class_code[k]->code;
-        classes[k] = batch_class(c, p.class_set[k], p.device, group_of(int(c.z), width[k]));
+        classes[k] = batch_class(c, p.class_set[k], p.device, group_at(int(c.z), width[k]));
     }
     std::vector<std::vector<uint32_t>> members(nc);
     for (size_t i = 0; i < p.count; ++i) members[p.cls_of[i]].push_back(uint32_t(i));
@@ -307,11 +328,16 @@ void configure(cf_batch_plan& p, const cf_batch_plan_config& cfg) {
         const size_t shared_off = layered_shared_offset(lds);
         lds = shared_off + kSharedBytes;
         int per_cu = 0;
-        check_hip(layered_occupancy(bgm, half, solo, zc, w, lds, &per_cu, nms), "occupancy");
+        bool dual = w == kDualWidth;
+        if (!dual) {
+            check_hip(layered_occupancy(bgm, half, solo, zc, w, lds, &per_cu, nms), "occupancy");
+            dual = two_sets(solo, half, w, per_cu);
+        }
+        if (dual) check_hip(layered_occupancy(bgm, half, solo, zc, kDualWidth, lds, &per_cu, nms, true), "occupancy");
         per_cu = std::max(per_cu, 1);
         if (cfg.grid_per_cu) per_cu = std::min(per_cu, int(cfg.grid_per_cu));
         const int grid = int(std::min<size_t>(sg.n_tasks, size_t(per_cu) * size_t(cus)));
-        sg.launch = LayeredLaunch{bgm, half, solo, zc, w, std::max(grid, 1), lds, nms};
+        sg.launch = LayeredLaunch{bgm, half, solo, zc, dual ? kDualWidth : w, std::max(grid, 1), lds, nms, dual};
         sg.args.shared_off = int32_t(shared_off);
         segs.push_back(sg);
     }
@@ -516,7 +542,7 @@ cf_status cf_batch_plan_dims_get(const cf_batch_plan* plan, cf_batch_plan_dims* 
                                   uint32_t(plan->count),
                                   uint32_t(plan->class_code.size()),
                                   uint32_t(plan->tasks.size()),
-                                  s0 ? uint32_t(s0->threads) : plan->cfg.threads,
+                                  s0 ? uint32_t(s0->launch.threads) : plan->cfg.threads,
                                   s0 ? uint32_t(s0->launch.grid) : 0u,
                                   s0 ? uint32_t(s0->launch.lds) : 0u,
                                   plan->half ? CF_MSG_F16 : CF_MSG_F32,
@@ -563,7 +589,7 @@ cf_status cf_batch_plan_segment_get(const cf_batch_plan* plan, uint32_t index, c
         if (index >= plan->segs.size())
             fail(Kind::Config, "launch " + std::to_string(index) + " of " + std::to_string(plan->segs.size()));
         const PlanSegment& s = plan->segs[index];
-        *out = cf_batch_plan_segment{uint32_t(s.threads), uint32_t(s.launch.grid), uint32_t(s.launch.lds),
+        *out = cf_batch_plan_segment{uint32_t(s.launch.threads), uint32_t(s.launch.grid), uint32_t(s.launch.lds),
                                      s.classes,           uint32_t(s.n_tasks),     s.codewords};
     });
 }

@@ -13,7 +13,7 @@ namespace {
 
 const void* select_kernel(int bg_mask, bool half, bool solo, int zc, bool nms, bool dual) {
     if (bg_mask < 1 || bg_mask > 3) return nullptr;
-    if (dual && (half || !solo || zc != 384 || bg_mask == 3)) return nullptr;
+    if (dual && (half || !solo || (zc == 384 && bg_mask == 3))) return nullptr;
     return half ? layered_kernel_f16(bg_mask, solo, zc, nms) : layered_kernel_f32(bg_mask, solo, zc, nms, dual);
 }
 

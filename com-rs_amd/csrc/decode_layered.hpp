@@ -789,8 +789,9 @@ __device__ __forceinline__ void place_direct(typename Store<HALF>::T* post, cons
 
 // One task: codewords order[first .. first + cnt) of class c (base graph BG). SOLO: the
 // task has one codeword (G = 1), whose posteriors start the LDS; ZC: Z, when every class of
-// the launch has the same (0: read at run time); DUAL: two sets of Z = 384 lanes on the
-// codeword, each running its layers (the second set places no LLRs and writes no outputs).
+// the launch has the same (0: read at run time); DUAL: two sets of 384 lanes on a task of one
+// codeword (FP32), each running its layers on rows 0 .. Z-1 (the second set places no LLRs
+// and writes no outputs).
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL>
 __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass& c, uint32_t first, int cnt, Shared& sh,
                          void* lds_dyn, uint32_t& stamp, uint32_t& round) {
@@ -799,13 +800,13 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
     const int z = ZC ? ZC : c.z, n = ZC ? Graph<BG>::cols * ZC : c.n;
     T* post_all = static_cast<T*>(lds_dyn);
     uint32_t* hd_all = reinterpret_cast<uint32_t*>(post_all + c.g * n);  // g n is even: 4-byte aligned
-    static_assert(!DUAL || (SOLO && ZC == 384 && !HALF), "two wave sets serve FP32 tasks of one codeword, Z = 384");
+    static_assert(!DUAL || (SOLO && !HALF), "two wave sets serve FP32 tasks of one codeword");
     const int tid = threadIdx.x;
-    // DUAL: the lane's set, uniform over a wavefront (Z = 384 is six of them)
-    const int set = DUAL ? int(__builtin_amdgcn_readfirstlane(uint32_t(tid >= ZC))) : 0;
-    const int g = SOLO ? 0 : tid / z, r = tid - g * z - set * z;
+    // DUAL: the lane's set, uniform over a wavefront (384 lanes are six of them)
+    const int set = DUAL ? int(__builtin_amdgcn_readfirstlane(uint32_t(tid >= kMaxThreads))) : 0;
+    const int g = SOLO ? 0 : tid / z, r = tid - g * z - set * kMaxThreads;
     // lanes beyond the task's codewords (SOLO: beyond Z) idle
-    const bool lane_ok = SOLO ? (DUAL || tid < z) : g < cnt;
+    const bool lane_ok = SOLO ? r < z : g < cnt;
     const bool places = !DUAL || set == 0;  // places the LLRs and writes the outputs
     const uint32_t id = lane_ok ? (a.order ? a.order[first + g] : first + uint32_t(g)) : 0u;
     T* post = post_all + (lane_ok ? g : 0) * n;
@@ -1011,9 +1012,9 @@ __global__ void __launch_bounds__(kMaxThreads, HALF ? 3 : 1) decode_layered_kern
 }
 
 // 768 lanes: twelve waves a CU, three on every SIMD (at most 168 registers a lane)
-template <int BG, bool NMS>
+template <int BGMASK, int ZC, bool NMS>
 __global__ void __launch_bounds__(2 * kMaxThreads, 1) decode_layered_dual_kernel(LayeredArgs a) {
-    layered_body<BG, false, true, 384, NMS, true>(a);
+    layered_body<BGMASK, false, true, ZC, NMS, true>(a);
 }
 
 }  // namespace layered

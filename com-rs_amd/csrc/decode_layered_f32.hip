@@ -7,11 +7,18 @@ namespace comfec {
 const void* layered_kernel_f32(int bg_mask, bool solo, int zc, bool nms, bool dual) {
     using namespace layered;
     if (dual) {
-        if (!solo || zc != 384) return nullptr;
-        if (bg_mask == 1) return nms ? reinterpret_cast<const void*>(decode_layered_dual_kernel<1, true>)
-                                     : reinterpret_cast<const void*>(decode_layered_dual_kernel<1, false>);
-        if (bg_mask == 2) return nms ? reinterpret_cast<const void*>(decode_layered_dual_kernel<2, true>)
-                                     : reinterpret_cast<const void*>(decode_layered_dual_kernel<2, false>);
+        if (!solo) return nullptr;
+        if (zc == 384) {
+            if (bg_mask == 1) return nms ? reinterpret_cast<const void*>(decode_layered_dual_kernel<1, 384, true>)
+                                         : reinterpret_cast<const void*>(decode_layered_dual_kernel<1, 384, false>);
+            if (bg_mask == 2) return nms ? reinterpret_cast<const void*>(decode_layered_dual_kernel<2, 384, true>)
+                                         : reinterpret_cast<const void*>(decode_layered_dual_kernel<2, 384, false>);
+            return nullptr;
+        }
+        if (zc != 0 || nms) return nullptr;
+        if (bg_mask == 1) return reinterpret_cast<const void*>(decode_layered_dual_kernel<1, 0, false>);
+        if (bg_mask == 2) return reinterpret_cast<const void*>(decode_layered_dual_kernel<2, 0, false>);
+        if (bg_mask == 3) return reinterpret_cast<const void*>(decode_layered_dual_kernel<3, 0, false>);
         return nullptr;
     }
     if (solo && zc == 384) {

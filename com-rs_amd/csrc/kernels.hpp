@@ -101,7 +101,7 @@ struct LayeredLaunch {
     int grid;
     size_t lds;                 // dynamic LDS: layered_shared_offset + kSharedBytes
     bool nms = false;           // every class the normalized min-sum and no learned parameters (solo, zc 384)
-    bool dual = false;          // two wave sets a codeword (FP32, solo, zc 384): 2 x 384 lanes
+    bool dual = false;          // two wave sets a codeword (FP32, solo): 2 x 384 lanes
 };
 
 bool layered_supported(int bg, int z);

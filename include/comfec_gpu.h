@@ -164,7 +164,7 @@ typedef struct {
     uint64_t llr_total;     /* llr_offsets[count] */
     uint64_t bits_total;    /* the sum of K' */
     uint32_t codewords, classes, tasks;
-    uint32_t threads;       /* lanes per workgroup */
+    uint32_t threads;       /* lanes per workgroup (768 where two wave sets share a codeword) */
     uint32_t grid;          /* workgroups of the launch (resident ones: the launch is persistent) */
     uint32_t lds_bytes;     /* dynamic LDS per workgroup */
     int msg_type;
@@ -179,7 +179,9 @@ typedef struct {
                                  (rounded up) to 384; ignored with per_code_width */
     uint32_t per_code_width;  /* 1: each code at the width (a multiple of 64 from its Z rounded
                                  up to 384) with the most codeword rows per busy SIMD at the
-                                 device's occupancy, one launch per width, the widest first */
+                                 device's occupancy, or, in FP32 where one codeword a task
+                                 alone fills a CU at that width, on two wave sets of 384
+                                 lanes; one launch per width, the widest first */
     uint32_t grid_per_cu;     /* resident workgroups per CU at most (0: as many as fit) */
     int msg_type;             /* CF_MSG_F32 or CF_MSG_F16 */
     uint32_t overlap;         /* 1: the launches of a run go on parallel streams forked from
@@ -187,7 +189,8 @@ typedef struct {
                                  take the CUs another's leave; 0: one after another */
 } cf_batch_plan_config;
 
-/* One launch of a plan's run. */
+/* One launch of a plan's run: its lanes per workgroup (twice 384 where FP32 tasks of one
+   codeword would leave one workgroup a CU: two wave sets on each codeword), grid and LDS. */
 typedef struct {
     uint32_t threads, grid, lds_bytes;
     uint32_t classes, tasks, codewords;
