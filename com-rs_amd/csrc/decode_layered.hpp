@@ -332,7 +332,7 @@ struct Messages<ROWS, true> {
 // funnel shift) and one index whose magnitude is g(m2); a message is rebuilt as that
 // magnitude with its sign bit replaced by the stored one, which is the output bit for bit.
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I>
-__device__ __forceinline__ void layer(uint32_t gofs, uint32_t rbp, const uint2* __restrict__ edge,
+__device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb, const uint2* __restrict__ edge,
                                       Messages<msg_slots<BG, DUAL>(), HALF>& msg, float a, float b) {
     using S = Store<HALF>;
     constexpr int SL = slot_of<BG, DUAL>(I);  // where the lane keeps this layer's messages
@@ -340,18 +340,18 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t rbp, const uint2* 
     constexpr int deg = row_begin<BG>(I + 1) - e0;
     static_assert(ZC == 0 || SOLO, "a compile-time Z serves tasks of one codeword");
     // Addresses are LDS byte addresses (the posteriors start at address 0). rbp: the row's
-    // byte offset in a column and that minus Z, in 16-bit halves; gofs: the byte offset of
-    // the lane's codeword. With Z known at compile time (ZC) the shifts and the column bases
-    // are constants too, so rbp is made opaque, or the compiler would compute all 316
-    // loop-invariant addresses once and hold them in registers; the column bases are then
+    // byte offset in a column and that minus Z, in 16-bit halves; rb: its low half; gofs: the
+    // byte offset of the lane's codeword. With Z known at compile time (ZC) the shifts and
+    // the column bases are constants too, so rbp and rb are made opaque, or the compiler
+    // would compute all 316 loop-invariant addresses once and hold them in registers (they
+    // pass from layer to layer by reference, so no copy is made); the column bases are then
     // the LDS instructions' immediate offsets.
     uint32_t base = 0;
-    if constexpr (ZC != 0) asm volatile("" : "+v"(rbp));
+    if constexpr (ZC != 0) asm volatile("" : "+v"(rbp), "+v"(rb));
     if constexpr (!SOLO) {
         base = gofs;
         asm volatile("" : "+v"(base));
     }
-    const uint32_t rb = rbp & 0xffffu;
     float g1o, g2o;
     msg.template get<SL>(g1o, g2o);
     const uint32_t pko = msg.pk[SL];
@@ -449,9 +449,9 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t rbp, const uint2* 
 }
 
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I>
-__device__ __forceinline__ void sweep(uint32_t gofs, uint32_t rbp, bool active, int set, const uint2* edge,
-                                      Messages<msg_slots<BG, DUAL>(), HALF>& msg, const float* ab, float a,
-                                      float b) {
+__device__ __forceinline__ void sweep(uint32_t gofs, uint32_t& rbp, uint32_t& rb, bool active, int set,
+                                      const uint2* edge, Messages<msg_slots<BG, DUAL>(), HALF>& msg, const float* ab,
+                                      float a, float b) {
     if constexpr (I < Graph<BG>::rows) {
         if (!NMS && ab) {
             a = ab[2 * I];
@@ -462,8 +462,8 @@ __device__ __forceinline__ void sweep(uint32_t gofs, uint32_t rbp, bool active, 
         // layers and spills hundreds of registers)
         // DUAL: set is uniform over a wavefront, so a wave of the other set skips the layer
         if ((!DUAL || set == Dual<BG>::p.owner[I]) && active)
-            layer<BG, HALF, SOLO, ZC, NMS, DUAL, I>(gofs, rbp, edge, msg, a, b);
-        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, I + 1>(gofs, rbp, active, set, edge, msg, ab, a, b);
+            layer<BG, HALF, SOLO, ZC, NMS, DUAL, I>(gofs, rbp, rb, edge, msg, a, b);
+        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, I + 1>(gofs, rbp, rb, active, set, edge, msg, ab, a, b);
     }
 }
 
@@ -814,7 +814,7 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
     const int limit = a.max_it ? int(a.max_it[id]) : c.max_iterations;
     // the row's byte offset in a column and that minus Z (mod 2^16), in 16-bit halves, and
     // the byte offset of the lane's codeword (the posteriors start at LDS address 0)
-    const uint32_t rbp = row_pair<HALF>(r, z);
+    uint32_t rbp = row_pair<HALF>(r, z), rb = rbp & 0xffffu;
     const uint32_t gbytes = uint32_t(lane_ok ? g : 0) * uint32_t(n) * uint32_t(sizeof(T));
 
     if (c.direct) place_direct<BG, HALF>(post, llr, c, r, z, lane_ok && places);
@@ -873,7 +873,7 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
         if (!any_lane(sh, !finished, round)) break;
         ++it;
         const float* ab = learned ? a.params + size_t(it - 1) * 2 * Graph<BG>::rows : nullptr;
-        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, 0>(gbytes, rbp, !finished, set, sh.edge, msg, ab, c.a, c.b);
+        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, 0>(gbytes, rbp, rb, !finished, set, sh.edge, msg, ab, c.a, c.b);
         __syncthreads();
         const bool fails = fails_now();
         if (!finished && !fails) {
