@@ -103,10 +103,11 @@ void run_layered(const cf_code* h, int device, const LayeredRun& r, hipStream_t 
     // sits 2:2:1:1 or 1:1:1:1)
     const bool dual = two_sets(solo, r.half, threads, per_cu);
     const int lanes = dual ? 2 * kMaxThreads : threads;
-    if (dual) check_hip(layered_occupancy(c.bg, r.half, solo, zc, lanes, lds, &per_cu, nms, true), "occupancy");
+    const size_t lds_run = dual ? lds + kExchangeBytes : lds;
+    if (dual) check_hip(layered_occupancy(c.bg, r.half, solo, zc, lanes, lds_run, &per_cu, nms, true), "occupancy");
     check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "device attribute");
     const int grid = int(std::min<int64_t>(a.n_tasks, int64_t(std::max(per_cu, 1)) * std::max(cus, 1)));
-    LayeredLaunch l{c.bg, r.half, solo, zc, lanes, std::max(grid, 1), lds, nms, dual};
+    LayeredLaunch l{c.bg, r.half, solo, zc, lanes, std::max(grid, 1), lds_run, nms, dual};
     if (counter) {
         // the caller's counter: a fill kernel (no memset node) zeroes it before the launch
         check_hip(launch_zero(reinterpret_cast<uint64_t*>(counter), 1, stream), "zero launch");
@@ -333,7 +334,10 @@ void configure(cf_batch_plan& p, const cf_batch_plan_config& cfg) {
             check_hip(layered_occupancy(bgm, half, solo, zc, w, lds, &per_cu, nms), "occupancy");
             dual = two_sets(solo, half, w, per_cu);
         }
-        if (dual) check_hip(layered_occupancy(bgm, half, solo, zc, kDualWidth, lds, &per_cu, nms, true), "occupancy");
+        if (dual) {
+            lds += kExchangeBytes;
+            check_hip(layered_occupancy(bgm, half, solo, zc, kDualWidth, lds, &per_cu, nms, true), "occupancy");
+        }
         per_cu = std::max(per_cu, 1);
         if (cfg.grid_per_cu) per_cu = std::min(per_cu, int(cfg.grid_per_cu));
         const int grid = int(std::min<size_t>(sg.n_tasks, size_t(per_cu) * size_t(cus)));

@@ -108,18 +108,22 @@ struct Sched {
     static constexpr Schedule<BG> s = make_schedule<BG>();
 };
 
-// Two sets of six waves on one codeword of Z = 384 (the DUAL instantiations, 768 lanes): a
-// CU places wave i and waves i + 4, i + 8 on one SIMD, so with set s = waves 6s .. 6s + 5
-// every SIMD holds waves of both sets. The layers between two barriers share no column, so
-// the layers of such a group are spread over the sets (the widest first, each to the set
-// with fewer edges of the group so far, ties to the set with fewer layers) and run side by
-// side; a group of one layer runs on one set while the other waits at the next barrier.
-// A lane keeps the messages of its set's layers only: slot[l] is layer l's place among
-// them, and slots the larger count.
+// Two sets of six waves on one codeword (the DUAL instantiations, 768 lanes): a CU places
+// wave i and waves i + 4, i + 8 on one SIMD, so with set s = waves 6s .. 6s + 5 every SIMD
+// holds waves of both sets. The layers between two barriers share no column, so the layers
+// of such a group are spread over the sets (the widest first, each to the set with fewer
+// edges of the group so far, ties to the set with fewer layers) and run side by side. A
+// group of one wide layer (kSplitDegree edges or more) is split: each set updates part of
+// every row's edges, and the two exchange their parts' two smallest magnitudes and sign
+// products through LDS; a group of one narrower layer runs on one set while the other waits
+// at the next barrier. A lane keeps the messages of its set's layers only: slot[s][l] is
+// layer l's place among set s's (-1: not its layer), and slots the larger count.
+constexpr int kSplitDegree = 12;
+
 template <int BG>
 struct DualPlan {
-    int owner[Graph<BG>::rows];
-    int slot[Graph<BG>::rows];
+    int owner[Graph<BG>::rows];  // 0, 1, or 2 for a layer split over both sets
+    int slot[2][Graph<BG>::rows];
     int slots;
 };
 
@@ -136,6 +140,13 @@ constexpr DualPlan<BG> make_dual() {
     for (int l = 0; l < R;) {
         int end = l + 1;
         while (end < R && !Sched<BG>::s.barrier[end]) ++end;
+        if (end == l + 1 && degree_of<BG>(l) >= kSplitDegree) {
+            d.owner[l] = 2;
+            ++count[0];
+            ++count[1];
+            l = end;
+            continue;
+        }
         int load[2] = {0, 0};
         bool placed[R] = {};
         for (int n = l; n < end; ++n) {
@@ -151,7 +162,8 @@ constexpr DualPlan<BG> make_dual() {
         l = end;
     }
     int next[2] = {0, 0};
-    for (int i = 0; i < R; ++i) d.slot[i] = next[d.owner[i]]++;
+    for (int i = 0; i < R; ++i)
+        for (int s = 0; s < 2; ++s) d.slot[s][i] = d.owner[i] == s || d.owner[i] == 2 ? next[s]++ : -1;
     d.slots = count[0] > count[1] ? count[0] : count[1];
     return d;
 }
@@ -169,7 +181,7 @@ constexpr int msg_slots() {
 }
 template <int BG, bool DUAL>
 constexpr int slot_of(int i) {
-    if constexpr (DUAL) return Dual<BG>::p.slot[i];
+    if constexpr (DUAL) return Dual<BG>::p.slot[Dual<BG>::p.owner[i] == 1 ? 1 : 0][i];
     else return i;
 }
 
@@ -331,14 +343,23 @@ struct Messages<ROWS, true> {
 // The stored state keeps the output sign bits (edge q at bit 5 + deg-1-q, shifted in with a
 // funnel shift) and one index whose magnitude is g(m2); a message is rebuilt as that
 // magnitude with its sign bit replaced by the stored one, which is the output bit for bit.
-template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I>
+//
+// The lane updates edges Q0 .. Q1-1 of its row (all of them, or with SPLIT its set's part of
+// a row split over both sets) and keeps their messages in slot SL. SPLIT: after its part's
+// inputs, the lane writes their two smallest magnitudes and sign product to LDS at xbase,
+// every lane of the workgroup (active or not) meets at one barrier, and the lane merges the
+// other set's part into its own (the minima as three inputs are merged above, the sign
+// products by XOR) before the outputs; its index of g(m2) is that of a part's edge whose
+// |input| is the merged m1, or 31 (none) if the other part holds it.
+template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I, int SL, int Q0, int Q1, bool SPLIT>
 __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb, const uint2* __restrict__ edge,
-                                      Messages<msg_slots<BG, DUAL>(), HALF>& msg, float a, float b) {
+                                      Messages<msg_slots<BG, DUAL>(), HALF>& msg, float a, float b, bool active,
+                                      uint32_t xbase, int set) {
     using S = Store<HALF>;
-    constexpr int SL = slot_of<BG, DUAL>(I);  // where the lane keeps this layer's messages
-    constexpr int e0 = row_begin<BG>(I);
-    constexpr int deg = row_begin<BG>(I + 1) - e0;
+    constexpr int e0 = row_begin<BG>(I) + Q0;
+    constexpr int deg = Q1 - Q0;  // the edges the lane updates
     static_assert(ZC == 0 || SOLO, "a compile-time Z serves tasks of one codeword");
+    static_assert(!SPLIT || (DUAL && !HALF), "split rows are a two-set FP32 layout");
     // Addresses are LDS byte addresses (the posteriors start at address 0). rbp: the row's
     // byte offset in a column and that minus Z, in 16-bit halves; rb: its low half; gofs: the
     // byte offset of the lane's codeword. With Z known at compile time (ZC) the shifts and
@@ -384,43 +405,64 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
     constexpr bool keep = !HALF || deg <= 10;
     float v2c[deg];
     uint32_t addr[keep ? deg : 1];
-#pragma unroll
-    for (int q = 0; q < deg; ++q) {
-        const uint32_t ad = address(q);
-        if constexpr (keep) addr[q] = ad;
-        const float mag = uint32_t(q) == i1o ? g2o : g1o;
-        // bit 31 from the stored word, the rest from the magnitude: (c ? a : b) bitwise
-        const float old = __uint_as_float(
-            __builtin_amdgcn_bitop3_b32(pko << (27 - deg + q), __float_as_uint(mag), 0x80000000u, 0xe4));
-        v2c[q] = load(ad) - old;
-    }
-    // the row's sign product: three-input XORs from the first input on
-    uint32_t parity = __float_as_uint(v2c[0]);
-#pragma unroll
-    for (int q = 1; q < deg; q += 2)
-        parity = q + 1 < deg ? __builtin_amdgcn_bitop3_b32(parity, __float_as_uint(v2c[q]),
-                                                           __float_as_uint(v2c[q + 1]), 0x96)
-                             : parity ^ __float_as_uint(v2c[q]);
-    // the two smallest magnitudes, three inputs at a time
     float m1 = INFINITY, m2 = INFINITY;
-    constexpr int triples = deg / 3;
+    uint32_t parity = 0;
+    if (!SPLIT || active) {
 #pragma unroll
-    for (int t = 0; t < triples; ++t) {
-        const float x = fabsf(v2c[3 * t]), y = fabsf(v2c[3 * t + 1]), w = fabsf(v2c[3 * t + 2]);
-        const float t1 = fminf(fminf(x, y), w), t2 = __builtin_amdgcn_fmed3f(x, y, w);
-        if (t == 0) {
-            m1 = t1;
-            m2 = t2;
-        } else {
-            m2 = __builtin_amdgcn_fmed3f(m1, t1, fminf(m2, t2));
-            m1 = fminf(m1, t1);
+        for (int q = 0; q < deg; ++q) {
+            const uint32_t ad = address(q);
+            if constexpr (keep) addr[q] = ad;
+            const float mag = uint32_t(q) == i1o ? g2o : g1o;
+            // bit 31 from the stored word, the rest from the magnitude: (c ? a : b) bitwise
+            const float old = __uint_as_float(
+                __builtin_amdgcn_bitop3_b32(pko << (27 - deg + q), __float_as_uint(mag), 0x80000000u, 0xe4));
+            v2c[q] = load(ad) - old;
+        }
+        // the row's sign product: three-input XORs from the first input on
+        parity = __float_as_uint(v2c[0]);
+#pragma unroll
+        for (int q = 1; q < deg; q += 2)
+            parity = q + 1 < deg ? __builtin_amdgcn_bitop3_b32(parity, __float_as_uint(v2c[q]),
+                                                               __float_as_uint(v2c[q + 1]), 0x96)
+                                 : parity ^ __float_as_uint(v2c[q]);
+        // the two smallest magnitudes, three inputs at a time
+        constexpr int triples = deg / 3;
+#pragma unroll
+        for (int t = 0; t < triples; ++t) {
+            const float x = fabsf(v2c[3 * t]), y = fabsf(v2c[3 * t + 1]), w = fabsf(v2c[3 * t + 2]);
+            const float t1 = fminf(fminf(x, y), w), t2 = __builtin_amdgcn_fmed3f(x, y, w);
+            if (t == 0) {
+                m1 = t1;
+                m2 = t2;
+            } else {
+                m2 = __builtin_amdgcn_fmed3f(m1, t1, fminf(m2, t2));
+                m1 = fminf(m1, t1);
+            }
+        }
+#pragma unroll
+        for (int q = 3 * triples; q < deg; ++q) {
+            const float m = fabsf(v2c[q]);
+            m2 = __builtin_amdgcn_fmed3f(m1, m2, m);
+            m1 = fminf(m1, m);
         }
     }
-#pragma unroll
-    for (int q = 3 * triples; q < deg; ++q) {
-        const float m = fabsf(v2c[q]);
-        m2 = __builtin_amdgcn_fmed3f(m1, m2, m);
-        m1 = fminf(m1, m);
+    if constexpr (SPLIT) {
+        // [m1 | m2 | sign product] x [set 0 | set 1] x Z rows of 4 bytes
+        constexpr uint32_t part = uint32_t(kMaxThreads) * 4u;
+        const uint32_t mine = xbase + uint32_t(set) * part + rb, other = xbase + uint32_t(1 - set) * part + rb;
+        if (active) {
+            *(lds_ptr<float>::type)(uintptr_t)mine = m1;
+            *(lds_ptr<float>::type)(uintptr_t)(mine + 2 * part) = m2;
+            *(lds_ptr<uint32_t>::type)(uintptr_t)(mine + 4 * part) = parity;
+        }
+        __syncthreads();
+        if (!active) return;
+        const float om1 = *(const lds_ptr<float>::type)(uintptr_t)other;
+        const float om2 = *(const lds_ptr<float>::type)(uintptr_t)(other + 2 * part);
+        const uint32_t opar = *(const lds_ptr<uint32_t>::type)(uintptr_t)(other + 4 * part);
+        m2 = __builtin_amdgcn_fmed3f(m1, om1, fminf(m2, om2));
+        m1 = fminf(m1, om1);
+        parity ^= opar;
     }
     // NMS: b = 0, and a max(m - 0, 0) is a m bit for bit for every magnitude m >= +0
     float g1 = NMS ? a * m1 : a * fmaxf(m1 - b, 0.0f);
@@ -432,7 +474,7 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
     uint32_t g1s = __float_as_uint(g1) ^ parity, g2s = __float_as_uint(g2) ^ parity;
     // opaque, or the compiler folds the sign product back into every edge
     asm volatile("" : "+v"(g1s), "+v"(g2s));
-    uint32_t signs = 0, i1 = 0;
+    uint32_t signs = 0, i1 = SPLIT ? 31u : 0u;
 #pragma unroll
     for (int q = 0; q < deg; ++q) {
         const bool first = fabsf(v2c[q]) == m1;
@@ -451,7 +493,7 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I>
 __device__ __forceinline__ void sweep(uint32_t gofs, uint32_t& rbp, uint32_t& rb, bool active, int set,
                                       const uint2* edge, Messages<msg_slots<BG, DUAL>(), HALF>& msg, const float* ab,
-                                      float a, float b) {
+                                      float a, float b, uint32_t xbase) {
     if constexpr (I < Graph<BG>::rows) {
         if (!NMS && ab) {
             a = ab[2 * I];
@@ -460,10 +502,23 @@ __device__ __forceinline__ void sweep(uint32_t gofs, uint32_t& rbp, uint32_t& rb
         if constexpr (Sched<BG>::s.barrier[I]) __syncthreads();
         // (the branch also bounds the scheduler's reach: without it the compiler interleaves
         // layers and spills hundreds of registers)
-        // DUAL: set is uniform over a wavefront, so a wave of the other set skips the layer
-        if ((!DUAL || set == Dual<BG>::p.owner[I]) && active)
-            layer<BG, HALF, SOLO, ZC, NMS, DUAL, I>(gofs, rbp, rb, edge, msg, a, b);
-        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, I + 1>(gofs, rbp, rb, active, set, edge, msg, ab, a, b);
+        constexpr int d = degree_of<BG>(I);
+        if constexpr (DUAL && Dual<BG>::p.owner[I] == 2) {
+            // split over both sets (each path holds one barrier, which every wave meets)
+            constexpr int h = (d + 1) / 2;
+            if (set == 0)
+                layer<BG, HALF, SOLO, ZC, NMS, DUAL, I, Dual<BG>::p.slot[0][I], 0, h, true>(gofs, rbp, rb, edge, msg, a, b,
+                                                                                         active, xbase, set);
+            else
+                layer<BG, HALF, SOLO, ZC, NMS, DUAL, I, Dual<BG>::p.slot[1][I], h, d, true>(gofs, rbp, rb, edge, msg, a, b,
+                                                                                         active, xbase, set);
+        } else {
+            // DUAL: set is uniform over a wavefront, so a wave of the other set skips the layer
+            if ((!DUAL || set == Dual<BG>::p.owner[I]) && active)
+                layer<BG, HALF, SOLO, ZC, NMS, DUAL, I, slot_of<BG, DUAL>(I), 0, d, false>(gofs, rbp, rb, edge, msg, a, b,
+                                                                                        true, xbase, set);
+        }
+        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, I + 1>(gofs, rbp, rb, active, set, edge, msg, ab, a, b, xbase);
     }
 }
 
@@ -873,7 +928,8 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
         if (!any_lane(sh, !finished, round)) break;
         ++it;
         const float* ab = learned ? a.params + size_t(it - 1) * 2 * Graph<BG>::rows : nullptr;
-        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, 0>(gbytes, rbp, rb, !finished, set, sh.edge, msg, ab, c.a, c.b);
+        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, 0>(gbytes, rbp, rb, !finished, set, sh.edge, msg, ab, c.a, c.b,
+                                                uint32_t(a.shared_off) + uint32_t(kSharedBytes));
         __syncthreads();
         const bool fails = fails_now();
         if (!finished && !fails) {

@@ -49,6 +49,9 @@ struct GenericArgs {
 constexpr int kMaxThreads = 384;               // lanes of a workgroup: one per row of a layer
 constexpr int kMaxGroup = kMaxThreads / 2;     // codewords of one task (the smallest Z is 2)
 constexpr size_t kSharedBytes = 8192;          // a workgroup's tables and flags, after its posteriors
+// what a two-set workgroup adds after its tables: per set and row, the two smallest magnitudes
+// and the sign product of its part of a row split over both sets
+constexpr size_t kExchangeBytes = 3 * 2 * kMaxThreads * 4;
 
 // One code of a decode batch, as the layered kernel sees it.
 struct BatchClass {
