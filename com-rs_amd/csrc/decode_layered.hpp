@@ -407,15 +407,26 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
     uint32_t addr[keep ? deg : 1];
     float m1 = INFINITY, m2 = INFINITY;
     uint32_t parity = 0;
+    // The sign-bit mask in a vector register: a bitwise operation with a scalar operand
+    // issues more slowly on CDNA4 than one on vector registers only.
+    uint32_t sgn = 0x80000000u;
+    asm volatile("" : "+v"(sgn));
     if (!SPLIT || active) {
+        // edge q's stored sign at bit 31 of the word doubled q times: a doubling is a
+        // 32-bit add, which issues at twice the rate of a shift left by an immediate
+        uint32_t w = pko << (27 - deg);
 #pragma unroll
         for (int q = 0; q < deg; ++q) {
             const uint32_t ad = address(q);
             if constexpr (keep) addr[q] = ad;
             const float mag = uint32_t(q) == i1o ? g2o : g1o;
             // bit 31 from the stored word, the rest from the magnitude: (c ? a : b) bitwise
-            const float old = __uint_as_float(
-                __builtin_amdgcn_bitop3_b32(pko << (27 - deg + q), __float_as_uint(mag), 0x80000000u, 0xe4));
+            const float old = __uint_as_float(__builtin_amdgcn_bitop3_b32(w, __float_as_uint(mag), sgn, 0xe4));
+            if (q + 1 < deg) {
+                uint32_t w2;
+                asm("v_add_u32 %0, %1, %1" : "=v"(w2) : "v"(w));
+                w = w2;
+            }
             v2c[q] = load(ad) - old;
         }
         // the row's sign product: three-input XORs from the first input on
@@ -481,7 +492,7 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
         const uint32_t sel = first ? g2s : g1s;
         i1 = first ? uint32_t(q) : i1;
         // (a & c) ^ b bitwise
-        const uint32_t ob = __builtin_amdgcn_bitop3_b32(__float_as_uint(v2c[q]), sel, 0x80000000u, 0x6c);
+        const uint32_t ob = __builtin_amdgcn_bitop3_b32(__float_as_uint(v2c[q]), sel, sgn, 0x6c);
         signs = __builtin_amdgcn_alignbit(signs, ob, 31u);
         const float out = __uint_as_float(ob);
         const uint32_t ad = keep ? addr[keep ? q : 0] : address(q);
