@@ -120,11 +120,22 @@ struct Sched {
 // layer l's place among set s's (-1: not its layer), and slots the larger count.
 constexpr int kSplitDegree = 12;
 
+//
+// Lookahead: while a group of one layer (owner set A) runs, the other set would only wait at
+// the next barrier. If the next group holds a layer J of that set, the set reads J's inputs
+// on the columns the running layer does not write (premask[J]: its edges there, most of J
+// for BG1's extension rows, which share one or two columns with the layer before), forms
+// their variable-to-check messages, sign product and two smallest magnitudes, and keeps them
+// in registers over the barrier; J then reads only the rest. Every posterior is still read
+// after the last write before it in the reference's layer order, and the two minima and the
+// sign product do not depend on the order of their inputs, so the results are unchanged.
 template <int BG>
 struct DualPlan {
     int owner[Graph<BG>::rows];  // 0, 1, or 2 for a layer split over both sets
     int slot[2][Graph<BG>::rows];
     int slots;
+    int pre_of[Graph<BG>::rows];        // the layer the idle set reads ahead during this one's group, or -1
+    uint32_t premask[Graph<BG>::rows];  // of a layer read ahead: its edges read before its group's barrier
 };
 
 template <int BG>
@@ -165,6 +176,32 @@ constexpr DualPlan<BG> make_dual() {
     for (int i = 0; i < R; ++i)
         for (int s = 0; s < 2; ++s) d.slot[s][i] = d.owner[i] == s || d.owner[i] == 2 ? next[s]++ : -1;
     d.slots = count[0] > count[1] ? count[0] : count[1];
+    // lookahead over groups of one layer
+    for (int i = 0; i < R; ++i) {
+        d.pre_of[i] = -1;
+        d.premask[i] = 0u;
+    }
+    for (int l = 0; l < R; ++l) {
+        const bool alone = (l + 1 == R || Sched<BG>::s.barrier[l + 1]) && (l == 0 || Sched<BG>::s.barrier[l]);
+        if (!alone || d.owner[l] == 2 || l + 1 == R) continue;
+        int end = l + 2;  // the next group: l + 1 .. end - 1
+        while (end < R && !Sched<BG>::s.barrier[end]) ++end;
+        for (int j = l + 1; j < end; ++j) {
+            if (d.owner[j] != 1 - d.owner[l]) continue;
+            uint32_t mask = 0u;
+            for (int q = 0; q < degree_of<BG>(j); ++q) {
+                bool shared = false;
+                for (int e = row_begin<BG>(l); e < row_begin<BG>(l + 1); ++e)
+                    shared = shared || Graph<BG>::t[e].j == Graph<BG>::t[row_begin<BG>(j) + q].j;
+                if (!shared) mask |= 1u << q;
+            }
+            if (mask != 0u) {
+                d.pre_of[l] = j;
+                d.premask[j] = mask;
+            }
+            break;
+        }
+    }
     return d;
 }
 
@@ -203,6 +240,12 @@ constexpr uint32_t shift_of(int e) {
 
 template <bool HALF>
 struct Store;
+
+// floats in global memory (address space 1)
+typedef const float __attribute__((address_space(1))) gfloat;
+__device__ __forceinline__ const gfloat* to_global(const float* p) {
+    return reinterpret_cast<const gfloat*>(reinterpret_cast<uintptr_t>(p));
+}
 
 // a pointer into LDS made from a byte address
 template <typename T>
@@ -324,6 +367,17 @@ struct Messages<ROWS, true> {
     }
 };
 
+// The inputs of a layer read ahead (two-set lookahead): per edge its variable-to-check
+// message and posterior address, and the partial sign product and two smallest magnitudes
+// over those edges.
+constexpr int kAheadMax = 16;
+struct Ahead {
+    float v[kAheadMax];
+    uint32_t ad[kAheadMax];
+    float m1, m2;
+    uint32_t par;
+};
+
 // ldpc.rs:522-531 for the row of lane r in layer I of the codeword whose posteriors are
 // `post`.
 //
@@ -351,10 +405,15 @@ struct Messages<ROWS, true> {
 // other set's part into its own (the minima as three inputs are merged above, the sign
 // products by XOR) before the outputs; its index of g(m2) is that of a part's edge whose
 // |input| is the merged m1, or 31 (none) if the other part holds it.
-template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I, int SL, int Q0, int Q1, bool SPLIT>
+//
+// PRE (two-set lookahead): the edges whose inputs were read ahead into `ahead` (their
+// messages, addresses, and partial sign product and minima are taken from there). AHEAD:
+// read the inputs of the PRE edges into `ahead` and return (no outputs).
+template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I, int SL, int Q0, int Q1, bool SPLIT,
+          uint32_t PRE = 0u, bool AHEAD = false>
 __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb, const uint2* __restrict__ edge,
                                       Messages<msg_slots<BG, DUAL>(), HALF>& msg, float a, float b, bool active,
-                                      uint32_t xbase, int set) {
+                                      uint32_t xbase, int set, Ahead& ahead) {
     using S = Store<HALF>;
     constexpr int e0 = row_begin<BG>(I) + Q0;
     constexpr int deg = Q1 - Q0;  // the edges the lane updates
@@ -411,52 +470,116 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
     // issues more slowly on CDNA4 than one on vector registers only.
     uint32_t sgn = 0x80000000u;
     asm volatile("" : "+v"(sgn));
+    static_assert(PRE == 0u || (DUAL && !HALF && !SPLIT && deg <= kAheadMax), "lookahead serves whole FP32 rows");
+    static_assert(!AHEAD || PRE != 0u, "a lookahead reads some edges");
     if (!SPLIT || active) {
-        // edge q's stored sign at bit 31 of the word doubled q times: a doubling is a
-        // 32-bit add, which issues at twice the rate of a shift left by an immediate
-        uint32_t w = pko << (27 - deg);
+        // edge q's stored sign at bit 31 of the word shifted left 27 - deg + q places; along
+        // a run of edges the shift is one doubling more per edge (a 32-bit add, which issues
+        // at twice the rate of a shift left by an immediate)
+        uint32_t w = 0u;
+        int wq = -2;  // the edge whose sign w holds at bit 31
 #pragma unroll
         for (int q = 0; q < deg; ++q) {
+            // AHEAD: the PRE edges only; otherwise the PRE edges come from `ahead`
+            if (((PRE >> q) & 1u) != (AHEAD ? 1u : 0u)) {
+                if constexpr (!AHEAD) {
+                    if constexpr (keep) addr[q] = ahead.ad[q];
+                    v2c[q] = ahead.v[q];
+                }
+                continue;
+            }
             const uint32_t ad = address(q);
             if constexpr (keep) addr[q] = ad;
             const float mag = uint32_t(q) == i1o ? g2o : g1o;
-            // bit 31 from the stored word, the rest from the magnitude: (c ? a : b) bitwise
-            const float old = __uint_as_float(__builtin_amdgcn_bitop3_b32(w, __float_as_uint(mag), sgn, 0xe4));
-            if (q + 1 < deg) {
+            if (wq == q - 1) {
                 uint32_t w2;
                 asm("v_add_u32 %0, %1, %1" : "=v"(w2) : "v"(w));
                 w = w2;
-            }
-            v2c[q] = load(ad) - old;
-        }
-        // the row's sign product: three-input XORs from the first input on
-        parity = __float_as_uint(v2c[0]);
-#pragma unroll
-        for (int q = 1; q < deg; q += 2)
-            parity = q + 1 < deg ? __builtin_amdgcn_bitop3_b32(parity, __float_as_uint(v2c[q]),
-                                                               __float_as_uint(v2c[q + 1]), 0x96)
-                                 : parity ^ __float_as_uint(v2c[q]);
-        // the two smallest magnitudes, three inputs at a time
-        constexpr int triples = deg / 3;
-#pragma unroll
-        for (int t = 0; t < triples; ++t) {
-            const float x = fabsf(v2c[3 * t]), y = fabsf(v2c[3 * t + 1]), w = fabsf(v2c[3 * t + 2]);
-            const float t1 = fminf(fminf(x, y), w), t2 = __builtin_amdgcn_fmed3f(x, y, w);
-            if (t == 0) {
-                m1 = t1;
-                m2 = t2;
             } else {
+                w = pko << (27 - deg + q);
+            }
+            wq = q;
+            // bit 31 from the stored word, the rest from the magnitude: (c ? a : b) bitwise
+            const float old = __uint_as_float(__builtin_amdgcn_bitop3_b32(w, __float_as_uint(mag), sgn, 0xe4));
+            v2c[q] = load(ad) - old;
+            if constexpr (AHEAD) {
+                ahead.ad[q] = ad;
+                ahead.v[q] = v2c[q];
+            }
+        }
+        if constexpr (PRE == 0u) {
+            // the row's sign product: three-input XORs from the first input on
+            parity = __float_as_uint(v2c[0]);
+#pragma unroll
+            for (int q = 1; q < deg; q += 2)
+                parity = q + 1 < deg ? __builtin_amdgcn_bitop3_b32(parity, __float_as_uint(v2c[q]),
+                                                                   __float_as_uint(v2c[q + 1]), 0x96)
+                                     : parity ^ __float_as_uint(v2c[q]);
+            // the two smallest magnitudes, three inputs at a time
+            constexpr int triples = deg / 3;
+#pragma unroll
+            for (int t = 0; t < triples; ++t) {
+                const float x = fabsf(v2c[3 * t]), y = fabsf(v2c[3 * t + 1]), w = fabsf(v2c[3 * t + 2]);
+                const float t1 = fminf(fminf(x, y), w), t2 = __builtin_amdgcn_fmed3f(x, y, w);
+                if (t == 0) {
+                    m1 = t1;
+                    m2 = t2;
+                } else {
+                    m2 = __builtin_amdgcn_fmed3f(m1, t1, fminf(m2, t2));
+                    m1 = fminf(m1, t1);
+                }
+            }
+#pragma unroll
+            for (int q = 3 * triples; q < deg; ++q) {
+                const float m = fabsf(v2c[q]);
+                m2 = __builtin_amdgcn_fmed3f(m1, m2, m);
+                m1 = fminf(m1, m);
+            }
+        } else {
+            // the inputs of one part of the edges (AHEAD: the PRE edges; otherwise the rest,
+            // merged into the part read ahead), the PRE edges' list compacted at compile time
+            constexpr uint32_t part = AHEAD ? PRE : (((1u << deg) - 1u) & ~PRE);
+            constexpr int np = __builtin_popcount(part);
+            float x[np > 0 ? np : 1];
+            int k = 0;
+#pragma unroll
+            for (int q = 0; q < deg; ++q)
+                if ((part >> q) & 1u) x[k++] = v2c[q];
+            if constexpr (AHEAD) {
+                parity = 0u;
+                m1 = INFINITY;
+                m2 = INFINITY;
+            } else {
+                parity = ahead.par;
+                m1 = ahead.m1;
+                m2 = ahead.m2;
+            }
+#pragma unroll
+            for (int q = 0; q + 1 < np; q += 2)
+                parity = __builtin_amdgcn_bitop3_b32(parity, __float_as_uint(x[q]), __float_as_uint(x[q + 1]), 0x96);
+            if constexpr ((np & 1) != 0) parity ^= __float_as_uint(x[np - 1]);
+            constexpr int triples = np / 3;
+#pragma unroll
+            for (int t = 0; t < triples; ++t) {
+                const float p = fabsf(x[3 * t]), y = fabsf(x[3 * t + 1]), z = fabsf(x[3 * t + 2]);
+                const float t1 = fminf(fminf(p, y), z), t2 = __builtin_amdgcn_fmed3f(p, y, z);
                 m2 = __builtin_amdgcn_fmed3f(m1, t1, fminf(m2, t2));
                 m1 = fminf(m1, t1);
             }
-        }
 #pragma unroll
-        for (int q = 3 * triples; q < deg; ++q) {
-            const float m = fabsf(v2c[q]);
-            m2 = __builtin_amdgcn_fmed3f(m1, m2, m);
-            m1 = fminf(m1, m);
+            for (int q = 3 * triples; q < np; ++q) {
+                const float m = fabsf(x[q]);
+                m2 = __builtin_amdgcn_fmed3f(m1, m2, m);
+                m1 = fminf(m1, m);
+            }
+            if constexpr (AHEAD) {
+                ahead.par = parity;
+                ahead.m1 = m1;
+                ahead.m2 = m2;
+            }
         }
     }
+    if constexpr (AHEAD) return;
     if constexpr (SPLIT) {
         // [m1 | m2 | sign product] x [set 0 | set 1] x Z rows of 4 bytes
         constexpr uint32_t part = uint32_t(kMaxThreads) * 4u;
@@ -501,35 +624,95 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
     msg.pk[SL] = (signs << 5) | i1;
 }
 
+// the end of the barrier group that starts at layer I (its layers share no column)
+template <int BG>
+constexpr int group_end(int i) {
+    int e = i + 1;
+    while (e < Graph<BG>::rows && !Sched<BG>::s.barrier[e]) ++e;
+    return e;
+}
+
+// the layer of group [I, E) whose inputs were read ahead, or -1
+template <int BG>
+constexpr int ahead_in(int i, int e) {
+    for (int l = i; l < e; ++l)
+        if (Dual<BG>::p.premask[l] != 0u) return l;
+    return -1;
+}
+
+// One layer's update by the lanes that own it (no barrier before it); (a, b): its g(m).
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I>
-__device__ __forceinline__ void sweep(uint32_t gofs, uint32_t& rbp, uint32_t& rb, bool active, int set,
-                                      const uint2* edge, Messages<msg_slots<BG, DUAL>(), HALF>& msg, const float* ab,
-                                      float a, float b, uint32_t xbase) {
-    if constexpr (I < Graph<BG>::rows) {
-        if (!NMS && ab) {
-            a = ab[2 * I];
-            b = ab[2 * I + 1];
-        }
-        if constexpr (Sched<BG>::s.barrier[I]) __syncthreads();
+__device__ __forceinline__ void one_layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb, bool active, int set,
+                                          const uint2* edge, Messages<msg_slots<BG, DUAL>(), HALF>& msg, float a,
+                                          float b, uint32_t xbase, Ahead& ahead) {
+    constexpr int d = degree_of<BG>(I);
+    if constexpr (DUAL && Dual<BG>::p.owner[I] == 2) {
+        // split over both sets (each path holds one barrier, which every wave meets)
+        constexpr int h = (d + 1) / 2;
+        if (set == 0)
+            layer<BG, HALF, SOLO, ZC, NMS, DUAL, I, Dual<BG>::p.slot[0][I], 0, h, true>(gofs, rbp, rb, edge, msg, a, b,
+                                                                                     active, xbase, set, ahead);
+        else
+            layer<BG, HALF, SOLO, ZC, NMS, DUAL, I, Dual<BG>::p.slot[1][I], h, d, true>(gofs, rbp, rb, edge, msg, a, b,
+                                                                                     active, xbase, set, ahead);
+    } else {
+        // DUAL: set is uniform over a wavefront, so a wave of the other set skips the layer
         // (the branch also bounds the scheduler's reach: without it the compiler interleaves
         // layers and spills hundreds of registers)
-        constexpr int d = degree_of<BG>(I);
-        if constexpr (DUAL && Dual<BG>::p.owner[I] == 2) {
-            // split over both sets (each path holds one barrier, which every wave meets)
-            constexpr int h = (d + 1) / 2;
-            if (set == 0)
-                layer<BG, HALF, SOLO, ZC, NMS, DUAL, I, Dual<BG>::p.slot[0][I], 0, h, true>(gofs, rbp, rb, edge, msg, a, b,
-                                                                                         active, xbase, set);
-            else
-                layer<BG, HALF, SOLO, ZC, NMS, DUAL, I, Dual<BG>::p.slot[1][I], h, d, true>(gofs, rbp, rb, edge, msg, a, b,
-                                                                                         active, xbase, set);
-        } else {
-            // DUAL: set is uniform over a wavefront, so a wave of the other set skips the layer
-            if ((!DUAL || set == Dual<BG>::p.owner[I]) && active)
-                layer<BG, HALF, SOLO, ZC, NMS, DUAL, I, slot_of<BG, DUAL>(I), 0, d, false>(gofs, rbp, rb, edge, msg, a, b,
-                                                                                        true, xbase, set);
+        constexpr uint32_t pre = DUAL ? Dual<BG>::p.premask[I] : 0u;
+        if ((!DUAL || set == Dual<BG>::p.owner[I]) && active)
+            layer<BG, HALF, SOLO, ZC, NMS, DUAL, I, slot_of<BG, DUAL>(I), 0, d, false, pre>(
+                gofs, rbp, rb, edge, msg, a, b, true, xbase, set, ahead);
+    }
+}
+
+// The layers I .. E-1 of the group that starts at G, except P, in order; la, lb: the (a, b)
+// of the group's layers.
+template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int G, int I, int E, int P>
+__device__ __forceinline__ void group_rest(uint32_t gofs, uint32_t& rbp, uint32_t& rb, bool active, int set,
+                                           const uint2* edge, Messages<msg_slots<BG, DUAL>(), HALF>& msg,
+                                           const float* la, const float* lb, uint32_t xbase, Ahead& ahead) {
+    if constexpr (I < E) {
+        if constexpr (I != P)
+            one_layer<BG, HALF, SOLO, ZC, NMS, DUAL, I>(gofs, rbp, rb, active, set, edge, msg, la[I - G], lb[I - G],
+                                                        xbase, ahead);
+        group_rest<BG, HALF, SOLO, ZC, NMS, DUAL, G, I + 1, E, P>(gofs, rbp, rb, active, set, edge, msg, la, lb, xbase,
+                                                                   ahead);
+    }
+}
+
+// The barrier groups from the one that starts at layer I to the end of the sweep. A layer read
+// ahead goes first in its group, so that no other layer's code lies between the read-ahead
+// inputs and their use (they would hold their registers through it).
+template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I>
+__device__ __forceinline__ void sweep(uint32_t gofs, uint32_t& rbp, uint32_t& rb, bool active, int set,
+                                      const uint2* edge, Messages<msg_slots<BG, DUAL>(), HALF>& msg, const gfloat* ab,
+                                      float a, float b, uint32_t xbase, Ahead& ahead) {
+    if constexpr (I < Graph<BG>::rows) {
+        constexpr int E = group_end<BG>(I);
+        constexpr int P = DUAL ? ahead_in<BG>(I, E) : -1;
+        // the group's learned parameters, read before the barrier so that their latency
+        // passes while the waves wait there
+        float la[E - I], lb[E - I];
+#pragma unroll
+        for (int l = I; l < E; ++l) {
+            la[l - I] = !NMS && ab ? ab[2 * l] : a;
+            lb[l - I] = !NMS && ab ? ab[2 * l + 1] : b;
         }
-        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, I + 1>(gofs, rbp, rb, active, set, edge, msg, ab, a, b, xbase);
+        if constexpr (Sched<BG>::s.barrier[I]) __syncthreads();
+        if constexpr (P >= 0)
+            one_layer<BG, HALF, SOLO, ZC, NMS, DUAL, P>(gofs, rbp, rb, active, set, edge, msg, la[P - I], lb[P - I],
+                                                        xbase, ahead);
+        group_rest<BG, HALF, SOLO, ZC, NMS, DUAL, I, I, E, P>(gofs, rbp, rb, active, set, edge, msg, la, lb, xbase,
+                                                              ahead);
+        if constexpr (DUAL && E == I + 1 && Dual<BG>::p.pre_of[I] >= 0) {
+            // the other set reads its layer of the next group ahead
+            constexpr int J = Dual<BG>::p.pre_of[I];
+            if (set != Dual<BG>::p.owner[I] && active)
+                layer<BG, HALF, SOLO, ZC, NMS, DUAL, J, slot_of<BG, DUAL>(J), 0, degree_of<BG>(J), false,
+                      Dual<BG>::p.premask[J], true>(gofs, rbp, rb, edge, msg, a, b, true, xbase, set, ahead);
+        }
+        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, E>(gofs, rbp, rb, active, set, edge, msg, ab, a, b, xbase, ahead);
     }
 }
 
@@ -853,6 +1036,39 @@ __device__ __forceinline__ void place_direct(typename Store<HALF>::T* post, cons
     }
 }
 
+// Values every lane holds alike, made scalar: loads from global memory inside the task
+// loop come back in vector registers (the loop also stores to global memory), and a
+// class's fields in vector registers would cost a register per column base in the sweep.
+__device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uni(float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); }
+template <typename T>
+__device__ __forceinline__ const T* uni(const T* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v)), hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+    return reinterpret_cast<const T*>((uint64_t(hi) << 32) | lo);
+}
+__device__ __forceinline__ BatchClass uni(const BatchClass& c) {
+    BatchClass u;
+    u.bg = uni(c.bg);
+    u.z = uni(c.z);
+    u.n = uni(c.n);
+    u.k = uni(c.k);
+    u.k_info = uni(c.k_info);
+    u.e = uni(c.e);
+    u.g = uni(c.g);
+    u.max_iterations = uni(c.max_iterations);
+    u.a = uni(c.a);
+    u.b = uni(c.b);
+    u.shift32 = uni(c.shift32);
+    u.col = uni(c.col);
+    u.row_start = uni(c.row_start);
+    u.src_start = uni(c.src_start);
+    u.src = uni(c.src);
+    u.map = uni(c.map);
+    u.direct = uni(c.direct);
+    return u;
+}
+
 // One task: codewords order[first .. first + cnt) of class c (base graph BG). SOLO: the
 // task has one codeword (G = 1), whose posteriors start the LDS; ZC: Z, when every class of
 // the launch has the same (0: read at run time); DUAL: two sets of 384 lanes on a task of one
@@ -897,6 +1113,7 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
 
     Messages<msg_slots<BG, DUAL>(), HALF> msg;
     msg.zero();
+    Ahead ahead;
     bool finished = !lane_ok, restarted = false;
     uint32_t its = 0, ok = 0;
     // the syndrome, the short test first
@@ -938,9 +1155,12 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
         }
         if (!any_lane(sh, !finished, round)) break;
         ++it;
-        const float* ab = learned ? a.params + size_t(it - 1) * 2 * Graph<BG>::rows : nullptr;
+        // uniform, and a global-memory pointer: the sweep's reads of the parameters are then
+        // vector-memory loads that only the parameters' own use waits for (a generic pointer
+        // would make them flat loads, which every barrier's LDS wait waits for too)
+        const gfloat* ab = learned ? to_global(uni(a.params + size_t(it - 1) * 2 * Graph<BG>::rows)) : nullptr;
         sweep<BG, HALF, SOLO, ZC, NMS, DUAL, 0>(gbytes, rbp, rb, !finished, set, sh.edge, msg, ab, c.a, c.b,
-                                                uint32_t(a.shared_off) + uint32_t(kSharedBytes));
+                                                uint32_t(a.shared_off) + uint32_t(kSharedBytes), ahead);
         __syncthreads();
         const bool fails = fails_now();
         if (!finished && !fails) {
@@ -965,39 +1185,6 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
     __syncthreads();  // the next task overwrites the posteriors
 }
 
-// Values every lane holds alike, made scalar: loads from global memory inside the task
-// loop come back in vector registers (the loop also stores to global memory), and a
-// class's fields in vector registers would cost a register per column base in the sweep.
-__device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
-__device__ __forceinline__ float uni(float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); }
-template <typename T>
-__device__ __forceinline__ const T* uni(const T* p) {
-    const uint64_t v = reinterpret_cast<uint64_t>(p);
-    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v)), hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
-    return reinterpret_cast<const T*>((uint64_t(hi) << 32) | lo);
-}
-__device__ __forceinline__ BatchClass uni(const BatchClass& c) {
-    BatchClass u;
-    u.bg = uni(c.bg);
-    u.z = uni(c.z);
-    u.n = uni(c.n);
-    u.k = uni(c.k);
-    u.k_info = uni(c.k_info);
-    u.e = uni(c.e);
-    u.g = uni(c.g);
-    u.max_iterations = uni(c.max_iterations);
-    u.a = uni(c.a);
-    u.b = uni(c.b);
-    u.shift32 = uni(c.shift32);
-    u.col = uni(c.col);
-    u.row_start = uni(c.row_start);
-    u.src_start = uni(c.src_start);
-    u.src = uni(c.src);
-    u.map = uni(c.map);
-    u.direct = uni(c.direct);
-    return u;
-}
-
 // Workgroups take tasks until none is left. BGMASK: the base graphs of the classes; SOLO:
 // every class has G = 1; ZC: every class has Z = ZC (0: any); NMS: every class runs the
 // normalized min-sum and no learned parameters are given; DUAL: two wave sets a codeword.
@@ -1009,6 +1196,8 @@ __device__ __forceinline__ void layered_body(LayeredArgs a) {
     Shared& sh = *reinterpret_cast<Shared*>(reinterpret_cast<char*>(lds_dyn) + a.shared_off);
     for (int i = threadIdx.x; i < kMaxGroup; i += blockDim.x) sh.fails[i] = 0u;
     if (threadIdx.x < 3) sh.vote[threadIdx.x] = 0u;
+
+
     uint32_t stamp = 0, round = 0;
     int loaded = -1;
     uint32_t t = blockIdx.x;
