@@ -378,6 +378,17 @@ struct Ahead {
     uint32_t par;
 };
 
+// Gives the read-ahead values of a layer (edges PRE) an undefined value with no instruction:
+// an empty asm with them as outputs. On the path of the lanes that do not read ahead, the
+// compiler would otherwise set each of them to zero, one move per register.
+template <uint32_t PRE>
+__device__ __forceinline__ void ahead_undefined(Ahead& ah) {
+#pragma unroll
+    for (int q = 0; q < kAheadMax; ++q)
+        if ((PRE >> q) & 1u) asm volatile("" : "=v"(ah.v[q]), "=v"(ah.ad[q]));
+    asm volatile("" : "=v"(ah.m1), "=v"(ah.m2), "=v"(ah.par));
+}
+
 // ldpc.rs:522-531 for the row of lane r in layer I of the codeword whose posteriors are
 // `post`.
 //
@@ -448,16 +459,21 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
         for (int q = 0; q < deg; ++q) ed[q] = edge[e0 + q];
         asm volatile("" ::: "memory");
     }
-    auto address = [&](int q) -> uint32_t {
-        const int e = e0 + q;
+    // the column's first posterior: with ZC a constant, which becomes the LDS instruction's
+    // immediate offset; the rest of the address is the rotated row
+    auto colbase = [&](int q) -> uint32_t {
+        if constexpr (ZC != 0) return uint32_t(Graph<BG>::t[e0 + q].j) * uint32_t(ZC) * es;
+        else return 0u;
+    };
+    auto rotated = [&](int q) -> uint32_t {
         if constexpr (ZC != 0) {
-            const uint32_t s_q = shift_of<BG, ZC>(e) * es;
-            const uint32_t rr = s_q == 0 ? rb : rotate_bytes(rbp, s_q | (s_q << 16));
-            return rr + uint32_t(Graph<BG>::t[e].j) * uint32_t(ZC) * es;
+            const uint32_t s_q = shift_of<BG, ZC>(e0 + q) * es;
+            return s_q == 0 ? rb : rotate_bytes(rbp, s_q | (s_q << 16));
         } else {
             return rotate_bytes(rbp, ed[ZC != 0 ? 0 : q].x) + ed[ZC != 0 ? 0 : q].y + base;
         }
     };
+    auto address = [&](int q) -> uint32_t { return rotated(q) + colbase(q); };
     auto load = [&](uint32_t ad) -> float { return S::ld_at(ad); };
     // FP16 kernels run three waves a SIMD (168 registers): their widest layers compute the
     // addresses twice rather than hold them
@@ -478,18 +494,31 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
         // at twice the rate of a shift left by an immediate)
         uint32_t w = 0u;
         int wq = -2;  // the edge whose sign w holds at bit 31
+        float raw[AHEAD ? deg : 1];
+        if constexpr (AHEAD) {
+#pragma unroll
+            for (int q = 0; q < deg; ++q) {
+                if (((PRE >> q) & 1u) == 0u) continue;
+                ahead.ad[q] = rotated(q);
+                raw[q] = load(ahead.ad[q] + colbase(q));
+            }
+            asm volatile("" ::: "memory");
+        }
 #pragma unroll
         for (int q = 0; q < deg; ++q) {
             // AHEAD: the PRE edges only; otherwise the PRE edges come from `ahead`
             if (((PRE >> q) & 1u) != (AHEAD ? 1u : 0u)) {
                 if constexpr (!AHEAD) {
-                    if constexpr (keep) addr[q] = ahead.ad[q];
+                    if constexpr (keep) addr[q] = ahead.ad[q] + colbase(q);
                     v2c[q] = ahead.v[q];
                 }
                 continue;
             }
-            const uint32_t ad = address(q);
-            if constexpr (keep) addr[q] = ad;
+            uint32_t ad = 0u;
+            if constexpr (!AHEAD) {
+                ad = address(q);
+                if constexpr (keep) addr[q] = ad;
+            }
             const float mag = uint32_t(q) == i1o ? g2o : g1o;
             if (wq == q - 1) {
                 uint32_t w2;
@@ -501,10 +530,11 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
             wq = q;
             // bit 31 from the stored word, the rest from the magnitude: (c ? a : b) bitwise
             const float old = __uint_as_float(__builtin_amdgcn_bitop3_b32(w, __float_as_uint(mag), sgn, 0xe4));
-            v2c[q] = load(ad) - old;
             if constexpr (AHEAD) {
-                ahead.ad[q] = ad;
+                v2c[q] = raw[q] - old;
                 ahead.v[q] = v2c[q];
+            } else {
+                v2c[q] = load(ad) - old;
             }
         }
         if constexpr (PRE == 0u) {
@@ -578,6 +608,18 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
                 ahead.m2 = m2;
             }
         }
+    }
+    else {
+        // SPLIT, lanes of a finished codeword: no inputs (their values are never used; an
+        // empty asm defines them with no instruction, where the compiler would set each
+        // register to zero)
+#pragma unroll
+        for (int q = 0; q < deg; ++q) asm volatile("" : "=v"(v2c[q]));
+        if constexpr (keep) {
+#pragma unroll
+            for (int q = 0; q < deg; ++q) asm volatile("" : "=v"(addr[q]));
+        }
+        asm volatile("" : "=v"(m1), "=v"(m2), "=v"(parity));
     }
     if constexpr (AHEAD) return;
     if constexpr (SPLIT) {
@@ -711,6 +753,8 @@ __device__ __forceinline__ void sweep(uint32_t gofs, uint32_t& rbp, uint32_t& rb
             if (set != Dual<BG>::p.owner[I] && active)
                 layer<BG, HALF, SOLO, ZC, NMS, DUAL, J, slot_of<BG, DUAL>(J), 0, degree_of<BG>(J), false,
                       Dual<BG>::p.premask[J], true>(gofs, rbp, rb, edge, msg, a, b, true, xbase, set, ahead);
+            else
+                ahead_undefined<Dual<BG>::p.premask[J]>(ahead);
         }
         sweep<BG, HALF, SOLO, ZC, NMS, DUAL, E>(gofs, rbp, rb, active, set, edge, msg, ab, a, b, xbase, ahead);
     }
