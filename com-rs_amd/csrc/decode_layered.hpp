@@ -378,6 +378,17 @@ struct Ahead {
     uint32_t par;
 };
 
+// Values the sweep keeps in vector registers. A VALU instruction with an operand in a scalar
+// register or an inline constant issues at about half the rate of one whose operands are all
+// vector registers (measured at three waves a SIMD: about 5.3 against 2.7 cycles a wave
+// instruction), so the sign mask, the zero of the offset's clamp and the class's (a, b) are
+// made once per task and held.
+struct LaneRegs {
+    uint32_t sgn;  // 0x80000000
+    float zero;    // +0.0
+    float a, b;    // the class's g(m) = a max(m - b, 0)
+};
+
 // Gives the read-ahead values of a layer (edges PRE) an undefined value with no instruction:
 // an empty asm with them as outputs. On the path of the lanes that do not read ahead, the
 // compiler would otherwise set each of them to zero, one move per register.
@@ -424,7 +435,7 @@ template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I, int 
           uint32_t PRE = 0u, bool AHEAD = false>
 __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb, const uint2* __restrict__ edge,
                                       Messages<msg_slots<BG, DUAL>(), HALF>& msg, float a, float b, bool active,
-                                      uint32_t xbase, int set, Ahead& ahead) {
+                                      uint32_t xbase, int set, const LaneRegs& lr, Ahead& ahead) {
     using S = Store<HALF>;
     constexpr int e0 = row_begin<BG>(I) + Q0;
     constexpr int deg = Q1 - Q0;  // the edges the lane updates
@@ -482,10 +493,7 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
     uint32_t addr[keep ? deg : 1];
     float m1 = INFINITY, m2 = INFINITY;
     uint32_t parity = 0;
-    // The sign-bit mask in a vector register: a bitwise operation with a scalar operand
-    // issues more slowly on CDNA4 than one on vector registers only.
-    uint32_t sgn = 0x80000000u;
-    asm volatile("" : "+v"(sgn));
+    const uint32_t sgn = lr.sgn;
     static_assert(PRE == 0u || (DUAL && !HALF && !SPLIT && deg <= kAheadMax), "lookahead serves whole FP32 rows");
     static_assert(!AHEAD || PRE != 0u, "a lookahead reads some edges");
     if (!SPLIT || active) {
@@ -641,8 +649,8 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
         parity ^= opar;
     }
     // NMS: b = 0, and a max(m - 0, 0) is a m bit for bit for every magnitude m >= +0
-    float g1 = NMS ? a * m1 : a * fmaxf(m1 - b, 0.0f);
-    float g2 = NMS ? a * m2 : a * fmaxf(m2 - b, 0.0f);
+    float g1 = NMS ? a * m1 : a * fmaxf(m1 - b, lr.zero);
+    float g2 = NMS ? a * m2 : a * fmaxf(m2 - b, lr.zero);
     msg.template put<SL>(g1, g2);
     // the magnitudes with the row's sign product applied once: an edge's output is then
     // its choice XOR its own input's sign bit, and the stored bit is the output's sign
@@ -686,17 +694,17 @@ constexpr int ahead_in(int i, int e) {
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I>
 __device__ __forceinline__ void one_layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb, bool active, int set,
                                           const uint2* edge, Messages<msg_slots<BG, DUAL>(), HALF>& msg, float a,
-                                          float b, uint32_t xbase, Ahead& ahead) {
+                                          float b, uint32_t xbase, const LaneRegs& lr, Ahead& ahead) {
     constexpr int d = degree_of<BG>(I);
     if constexpr (DUAL && Dual<BG>::p.owner[I] == 2) {
         // split over both sets (each path holds one barrier, which every wave meets)
         constexpr int h = (d + 1) / 2;
         if (set == 0)
             layer<BG, HALF, SOLO, ZC, NMS, DUAL, I, Dual<BG>::p.slot[0][I], 0, h, true>(gofs, rbp, rb, edge, msg, a, b,
-                                                                                     active, xbase, set, ahead);
+                                                                                     active, xbase, set, lr, ahead);
         else
             layer<BG, HALF, SOLO, ZC, NMS, DUAL, I, Dual<BG>::p.slot[1][I], h, d, true>(gofs, rbp, rb, edge, msg, a, b,
-                                                                                     active, xbase, set, ahead);
+                                                                                     active, xbase, set, lr, ahead);
     } else {
         // DUAL: set is uniform over a wavefront, so a wave of the other set skips the layer
         // (the branch also bounds the scheduler's reach: without it the compiler interleaves
@@ -704,7 +712,7 @@ __device__ __forceinline__ void one_layer(uint32_t gofs, uint32_t& rbp, uint32_t
         constexpr uint32_t pre = DUAL ? Dual<BG>::p.premask[I] : 0u;
         if ((!DUAL || set == Dual<BG>::p.owner[I]) && active)
             layer<BG, HALF, SOLO, ZC, NMS, DUAL, I, slot_of<BG, DUAL>(I), 0, d, false, pre>(
-                gofs, rbp, rb, edge, msg, a, b, true, xbase, set, ahead);
+                gofs, rbp, rb, edge, msg, a, b, true, xbase, set, lr, ahead);
     }
 }
 
@@ -713,13 +721,14 @@ __device__ __forceinline__ void one_layer(uint32_t gofs, uint32_t& rbp, uint32_t
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int G, int I, int E, int P>
 __device__ __forceinline__ void group_rest(uint32_t gofs, uint32_t& rbp, uint32_t& rb, bool active, int set,
                                            const uint2* edge, Messages<msg_slots<BG, DUAL>(), HALF>& msg,
-                                           const float* la, const float* lb, uint32_t xbase, Ahead& ahead) {
+                                           const float* la, const float* lb, uint32_t xbase, const LaneRegs& lr,
+                                           Ahead& ahead) {
     if constexpr (I < E) {
         if constexpr (I != P)
             one_layer<BG, HALF, SOLO, ZC, NMS, DUAL, I>(gofs, rbp, rb, active, set, edge, msg, la[I - G], lb[I - G],
-                                                        xbase, ahead);
+                                                        xbase, lr, ahead);
         group_rest<BG, HALF, SOLO, ZC, NMS, DUAL, G, I + 1, E, P>(gofs, rbp, rb, active, set, edge, msg, la, lb, xbase,
-                                                                   ahead);
+                                                                   lr, ahead);
     }
 }
 
@@ -729,7 +738,7 @@ __device__ __forceinline__ void group_rest(uint32_t gofs, uint32_t& rbp, uint32_
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I>
 __device__ __forceinline__ void sweep(uint32_t gofs, uint32_t& rbp, uint32_t& rb, bool active, int set,
                                       const uint2* edge, Messages<msg_slots<BG, DUAL>(), HALF>& msg, const gfloat* ab,
-                                      float a, float b, uint32_t xbase, Ahead& ahead) {
+                                      float a, float b, uint32_t xbase, const LaneRegs& lr, Ahead& ahead) {
     if constexpr (I < Graph<BG>::rows) {
         constexpr int E = group_end<BG>(I);
         constexpr int P = DUAL ? ahead_in<BG>(I, E) : -1;
@@ -744,19 +753,19 @@ __device__ __forceinline__ void sweep(uint32_t gofs, uint32_t& rbp, uint32_t& rb
         if constexpr (Sched<BG>::s.barrier[I]) __syncthreads();
         if constexpr (P >= 0)
             one_layer<BG, HALF, SOLO, ZC, NMS, DUAL, P>(gofs, rbp, rb, active, set, edge, msg, la[P - I], lb[P - I],
-                                                        xbase, ahead);
+                                                        xbase, lr, ahead);
         group_rest<BG, HALF, SOLO, ZC, NMS, DUAL, I, I, E, P>(gofs, rbp, rb, active, set, edge, msg, la, lb, xbase,
-                                                              ahead);
+                                                              lr, ahead);
         if constexpr (DUAL && E == I + 1 && Dual<BG>::p.pre_of[I] >= 0) {
             // the other set reads its layer of the next group ahead
             constexpr int J = Dual<BG>::p.pre_of[I];
             if (set != Dual<BG>::p.owner[I] && active)
                 layer<BG, HALF, SOLO, ZC, NMS, DUAL, J, slot_of<BG, DUAL>(J), 0, degree_of<BG>(J), false,
-                      Dual<BG>::p.premask[J], true>(gofs, rbp, rb, edge, msg, a, b, true, xbase, set, ahead);
+                      Dual<BG>::p.premask[J], true>(gofs, rbp, rb, edge, msg, a, b, true, xbase, set, lr, ahead);
             else
                 ahead_undefined<Dual<BG>::p.premask[J]>(ahead);
         }
-        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, E>(gofs, rbp, rb, active, set, edge, msg, ab, a, b, xbase, ahead);
+        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, E>(gofs, rbp, rb, active, set, edge, msg, ab, a, b, xbase, lr, ahead);
     }
 }
 
@@ -1157,6 +1166,8 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
 
     Messages<msg_slots<BG, DUAL>(), HALF> msg;
     msg.zero();
+    LaneRegs lr{0x80000000u, 0.0f, c.a, c.b};
+    asm volatile("" : "+v"(lr.sgn), "+v"(lr.zero), "+v"(lr.a), "+v"(lr.b));
     Ahead ahead;
     bool finished = !lane_ok, restarted = false;
     uint32_t its = 0, ok = 0;
@@ -1203,8 +1214,8 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
         // vector-memory loads that only the parameters' own use waits for (a generic pointer
         // would make them flat loads, which every barrier's LDS wait waits for too)
         const gfloat* ab = learned ? to_global(uni(a.params + size_t(it - 1) * 2 * Graph<BG>::rows)) : nullptr;
-        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, 0>(gbytes, rbp, rb, !finished, set, sh.edge, msg, ab, c.a, c.b,
-                                                uint32_t(a.shared_off) + uint32_t(kSharedBytes), ahead);
+        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, 0>(gbytes, rbp, rb, !finished, set, sh.edge, msg, ab, lr.a, lr.b,
+                                                uint32_t(a.shared_off) + uint32_t(kSharedBytes), lr, ahead);
         __syncthreads();
         const bool fails = fails_now();
         if (!finished && !fails) {
