@@ -52,10 +52,12 @@ hipError_t layered_prepare(const LayeredLaunch& l) {
 
 hipError_t launch_layered(const LayeredArgs& a, const LayeredLaunch& l, hipStream_t stream, bool prepare) {
     const void* k = select_kernel(l.bg_mask, l.half, l.solo, l.zc, l.nms, l.dual);
-    // two wave sets take exactly 2 x 384 lanes
+    // two wave sets take exactly 2 x 384 lanes; a compile-time Z of 384 exactly 384 a set (the
+    // kernel then takes every lane to own a row of the task's codeword)
     if (!k || l.grid < 1 || (l.dual ? l.threads != 2 * kMaxThreads
                                     : (l.threads < 64 || l.threads > kMaxThreads || l.threads % 64 != 0)))
         return hipErrorInvalidValue;
+    if (l.zc == kMaxThreads && !l.dual && l.threads != kMaxThreads) return hipErrorInvalidValue;
     if (l.lds > kLdsBudget) return hipErrorInvalidValue;
     if (prepare) {
         // the dynamic LDS of this launch (the static tables come on top of it)

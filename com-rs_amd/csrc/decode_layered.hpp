@@ -1187,6 +1187,11 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
     // normalized min-sum (b = 0)
     bool learned = !NMS && a.params != nullptr;
     int it = 0, offset = 0;
+    // Z equal to the lanes of a set: every lane owns a row of the task's one codeword, and
+    // `finished` is the same in every lane (it changes only on uniform conditions: the
+    // syndrome's verdict, the iteration count, the codeword's limit), so the loop needs no
+    // vote across the workgroup (a barrier an iteration)
+    constexpr bool kUniform = SOLO && ZC == kMaxThreads;
     for (;;) {
         if (learned && it >= a.learned_iterations) {
             // the end of the learned iterations, the same for every codeword of the task
@@ -1195,6 +1200,8 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
                     restarted = true;
                     if (places) place_llrs<BG, HALF>(post, llr, c, r, z);
                     msg.zero();
+                    // no vote below: the placement's own barrier before the sweep reads it
+                    if constexpr (kUniform) __syncthreads();
                 } else {
                     finished = true;
                     its = uint32_t(it);
@@ -1208,7 +1215,8 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
             finished = true;
             its = uint32_t(offset + it);
         }
-        if (!any_lane(sh, !finished, round)) break;
+        // (uniform: read from the first lane, so that the loop's exit is a scalar branch)
+        if (kUniform ? __builtin_amdgcn_readfirstlane(uint32_t(finished)) != 0u : !any_lane(sh, !finished, round)) break;
         ++it;
         // uniform, and a global-memory pointer: the sweep's reads of the parameters are then
         // vector-memory loads that only the parameters' own use waits for (a generic pointer
