@@ -306,9 +306,10 @@ __device__ __forceinline__ uint32_t row_pair(int r, int z) {
 }
 
 // The messages of one lane's rows: output magnitudes g(m1), g(m2) and, packed, the
-// index of the edge that takes g(m2) (bits 0..4) and the output sign bits (edge q of a row
-// of degree deg at bit 5 + deg - 1 - q). The index in the low bits is built from inline
-// constants and compared with them, so no register holds an edge number.
+// index of the edge that takes g(m2) (bits 0..4) and the output sign bits (edge q of a
+// row at bit 31 - q, so the first edge's is the sign bit itself and each doubling of the word
+// brings the next one there). The index in the low bits is built from inline constants and
+// compared with them, so no register holds an edge number.
 template <int ROWS, bool HALF>
 struct Messages;
 
@@ -416,7 +417,7 @@ __device__ __forceinline__ void ahead_undefined(Ahead& ah) {
 //    m2 = m1); they are selected, never computed, so the order of the comparisons does not
 //    matter: three inputs at a time give their min3 and med3, and a pair (t1 <= t2) merges
 //    into (m1 <= m2) as min(m1, t1) and med3(m1, t1, min(m2, t2)).
-// The stored state keeps the output sign bits (edge q at bit 5 + deg-1-q, shifted in with a
+// The stored state keeps the output sign bits (edge q at bit 31 - q, shifted in with a
 // funnel shift) and one index whose magnitude is g(m2); a message is rebuilt as that
 // magnitude with its sign bit replaced by the stored one, which is the output bit for bit.
 //
@@ -457,7 +458,10 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
     float g1o, g2o;
     msg.template get<SL>(g1o, g2o);
     const uint32_t pko = msg.pk[SL];
-    const uint32_t i1o = pko & 31u;
+    // extracted once a row (opaque, or each edge's compare becomes a byte-select form that
+    // issues more slowly than the mask and plain compares together)
+    uint32_t i1o = pko & 31u;
+    asm volatile("" : "+v"(i1o));
     constexpr uint32_t es = sizeof(typename S::T);
     // The posterior of entry e on the lane's row: (r + shift) mod Z in column j. With ZC a
     // zero shift (every extension column) needs no wrap. With Z known only at run time the
@@ -497,9 +501,9 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
     static_assert(PRE == 0u || (DUAL && !HALF && !SPLIT && deg <= kAheadMax), "lookahead serves whole FP32 rows");
     static_assert(!AHEAD || PRE != 0u, "a lookahead reads some edges");
     if (!SPLIT || active) {
-        // edge q's stored sign at bit 31 of the word shifted left 27 - deg + q places; along
-        // a run of edges the shift is one doubling more per edge (a 32-bit add, which issues
-        // at twice the rate of a shift left by an immediate)
+        // edge q's stored sign at bit 31 of the word shifted left q places; along a run of
+        // edges the shift is one doubling more per edge (a 32-bit add of vector registers,
+        // which issues at twice the rate of a shift left by an immediate)
         uint32_t w = 0u;
         int wq = -2;  // the edge whose sign w holds at bit 31
         float raw[AHEAD ? deg : 1];
@@ -533,7 +537,7 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
                 asm("v_add_u32 %0, %1, %1" : "=v"(w2) : "v"(w));
                 w = w2;
             } else {
-                w = pko << (27 - deg + q);
+                w = pko << q;
             }
             wq = q;
             // bit 31 from the stored word, the rest from the magnitude: (c ? a : b) bitwise
@@ -671,7 +675,7 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
         const uint32_t ad = keep ? addr[keep ? q : 0] : address(q);
         S::st_at(ad, v2c[q] + out);
     }
-    msg.pk[SL] = (signs << 5) | i1;
+    msg.pk[SL] = (signs << (32 - deg)) | i1;
 }
 
 // the end of the barrier group that starts at layer I (its layers share no column)
