@@ -31,11 +31,10 @@ LDS_PUBLISHED_GBS = 256 * 128 * 2.4
 VALU_PUBLISHED_GINST = 1024 * 2.4 / 4
 # What the profiles of the plain launch on this workload measured, per codeword
 # (profiles/decode_layered_bg1_z384_16qam.md): HBM bytes from rocprofv3 FETCH_SIZE (x 2.0)
-# and WRITE_SIZE (x 1.0), the channel LLRs in, the decisions out and the scratch of the
-# registers spilled in the task set-up (498.9 MB per 4096 codewords); SQ_INSTS_VALU per
-# codeword and iteration (both wave sets, 12 waves)
-TRAFFIC_PER_CODEWORD = 121800
-VALU_PER_CODEWORD_ITERATION = 34000
+# and WRITE_SIZE (x 1.0), the channel LLRs in and the decisions out (451.9 MB per 4096
+# codewords); SQ_INSTS_VALU per codeword and iteration (both wave sets, 12 waves)
+TRAFFIC_PER_CODEWORD = 110316
+VALU_PER_CODEWORD_ITERATION = 32231
 PROFILE = "profiles/decode_layered_bg1_z384_16qam.md"
 LEARNED_ITERATIONS = 10
 K_INFO, E, Z, BG1_EDGES = 8448, 25344, 384, 316
