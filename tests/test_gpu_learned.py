@@ -32,6 +32,30 @@ def test_plain_parameters_decode_as_the_plain_kernel():
     assert torch.equal(bits, bits2) and torch.equal(its, its2)
 
 
+def test_bg1_z384_plain_parameters_and_fallback_decode_as_the_plain_kernel():
+    """BG1 Z=384 (the two-wave-set kernel with Z compiled in, whose iteration loop takes
+    the codeword's state from its first lane): the learned launch with the plain parameters
+    (a = 0.75, b = 0) equals the plain normalized min-sum launch bit for bit; with T = 3 and
+    the fallback, a frame that the three learned iterations do not finish is decoded again
+    from its LLRs by the plain rule, so every frame's decisions are the plain decoder's, and
+    its iteration count is the plain count (at most 3) or 3 plus it."""
+    print("\n[learned] BG1 Z=384 16-QAM: plain parameters, T = 3 with fallback", flush=True)
+    code = F.NrLdpc(8448, 25344, 1, 0, 4).set_decoder("normalized_min_sum", 0.75, True, 10)
+    llr, _ = F.generate_batch(code, 512, F.QAM16, 4.2, snr_kind=F.ESN0_DB, seed=21)
+    bits, its, ok = code.decode_batch(llr)
+    bits2, its2, ok2 = F.learned_decode_batch(code, F.initial_params(code, 10), llr, fallback=False)
+    assert torch.equal(bits, bits2) and torch.equal(its, its2)
+    T = 3
+    bits3, its3, ok3 = F.learned_decode_batch(code, F.initial_params(code, T), llr, fallback=True)
+    assert torch.equal(bits3, bits)
+    p = its.cpu().numpy()
+    want = np.where(p <= T, p, T + p)
+    assert np.array_equal(its3.cpu().numpy(), want)
+    assert np.count_nonzero(p > T) > 100  # the fallback path runs on most frames
+    # ok: finished within the learned iterations
+    assert np.array_equal(ok3.cpu().numpy().astype(bool), p <= T)
+
+
 @pytest.mark.parametrize("bg,z,T", [(2, 52, 4), (1, 16, 3)])
 def test_loss_and_gradient_match_the_oracle(bg, z, T):
     print(f"\n[learned] gradient vs oracle BG{bg} Z={z} T={T}", flush=True)
