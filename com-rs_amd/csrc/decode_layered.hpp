@@ -388,6 +388,7 @@ struct LaneRegs {
     uint32_t sgn;  // 0x80000000
     float zero;    // +0.0
     float a, b;    // the class's g(m) = a max(m - b, 0)
+    uint32_t rbh;  // the lane's row offset + 64 KiB: a base for columns past the LDS offset field
 };
 
 // Gives the read-ahead values of a layer (edges PRE) an undefined value with no instruction:
@@ -436,7 +437,7 @@ template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I, int 
           uint32_t PRE = 0u, bool AHEAD = false>
 __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb, const uint2* __restrict__ edge,
                                       Messages<msg_slots<BG, DUAL>(), HALF>& msg, float a, float b, bool active,
-                                      uint32_t xbase, int set, const LaneRegs& lr, Ahead& ahead) {
+                                      uint32_t xbase, int set, LaneRegs& lr, Ahead& ahead) {
     using S = Store<HALF>;
     constexpr int e0 = row_begin<BG>(I) + Q0;
     constexpr int deg = Q1 - Q0;  // the edges the lane updates
@@ -476,14 +477,24 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
     }
     // the column's first posterior: with ZC a constant, which becomes the LDS instruction's
     // immediate offset; the rest of the address is the rotated row
+    // (a column whose base exceeds the 16-bit offset field, with no shift, takes its offset
+    // from the lane's second base, 64 KiB above its row)
+    // made opaque in place, as rb is
+    if constexpr (ZC != 0) asm volatile("" : "+v"(lr.rbh));
+    const uint32_t rbh = lr.rbh;
+    auto high = [&](int q) -> bool {
+        if constexpr (ZC != 0)
+            return shift_of<BG, ZC>(e0 + q) == 0u && uint32_t(Graph<BG>::t[e0 + q].j) * uint32_t(ZC) * es >= 65536u;
+        else return false;
+    };
     auto colbase = [&](int q) -> uint32_t {
-        if constexpr (ZC != 0) return uint32_t(Graph<BG>::t[e0 + q].j) * uint32_t(ZC) * es;
+        if constexpr (ZC != 0) return uint32_t(Graph<BG>::t[e0 + q].j) * uint32_t(ZC) * es - (high(q) ? 65536u : 0u);
         else return 0u;
     };
     auto rotated = [&](int q) -> uint32_t {
         if constexpr (ZC != 0) {
             const uint32_t s_q = shift_of<BG, ZC>(e0 + q) * es;
-            return s_q == 0 ? rb : rotate_bytes(rbp, s_q | (s_q << 16));
+            return s_q == 0 ? (high(q) ? rbh : rb) : rotate_bytes(rbp, s_q | (s_q << 16));
         } else {
             return rotate_bytes(rbp, ed[ZC != 0 ? 0 : q].x) + ed[ZC != 0 ? 0 : q].y + base;
         }
@@ -658,8 +669,9 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
     msg.template put<SL>(g1, g2);
     // the magnitudes with the row's sign product applied once: an edge's output is then
     // its choice XOR its own input's sign bit, and the stored bit is the output's sign
-    parity &= 0x80000000u;
-    uint32_t g1s = __float_as_uint(g1) ^ parity, g2s = __float_as_uint(g2) ^ parity;
+    // (parity & sign mask) ^ magnitude, one three-input operation each on vector registers
+    uint32_t g1s = __builtin_amdgcn_bitop3_b32(parity, __float_as_uint(g1), sgn, 0x6c),
+             g2s = __builtin_amdgcn_bitop3_b32(parity, __float_as_uint(g2), sgn, 0x6c);
     // opaque, or the compiler folds the sign product back into every edge
     asm volatile("" : "+v"(g1s), "+v"(g2s));
     uint32_t signs = 0, i1 = SPLIT ? 31u : 0u;
@@ -698,7 +710,7 @@ constexpr int ahead_in(int i, int e) {
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I>
 __device__ __forceinline__ void one_layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb, bool active, int set,
                                           const uint2* edge, Messages<msg_slots<BG, DUAL>(), HALF>& msg, float a,
-                                          float b, uint32_t xbase, const LaneRegs& lr, Ahead& ahead) {
+                                          float b, uint32_t xbase, LaneRegs& lr, Ahead& ahead) {
     constexpr int d = degree_of<BG>(I);
     if constexpr (DUAL && Dual<BG>::p.owner[I] == 2) {
         // split over both sets (each path holds one barrier, which every wave meets)
@@ -725,7 +737,7 @@ __device__ __forceinline__ void one_layer(uint32_t gofs, uint32_t& rbp, uint32_t
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int G, int I, int E, int P>
 __device__ __forceinline__ void group_rest(uint32_t gofs, uint32_t& rbp, uint32_t& rb, bool active, int set,
                                            const uint2* edge, Messages<msg_slots<BG, DUAL>(), HALF>& msg,
-                                           const float* la, const float* lb, uint32_t xbase, const LaneRegs& lr,
+                                           const float* la, const float* lb, uint32_t xbase, LaneRegs& lr,
                                            Ahead& ahead) {
     if constexpr (I < E) {
         if constexpr (I != P)
@@ -742,7 +754,7 @@ __device__ __forceinline__ void group_rest(uint32_t gofs, uint32_t& rbp, uint32_
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I>
 __device__ __forceinline__ void sweep(uint32_t gofs, uint32_t& rbp, uint32_t& rb, bool active, int set,
                                       const uint2* edge, Messages<msg_slots<BG, DUAL>(), HALF>& msg, const gfloat* ab,
-                                      float a, float b, uint32_t xbase, const LaneRegs& lr, Ahead& ahead) {
+                                      float a, float b, uint32_t xbase, LaneRegs& lr, Ahead& ahead) {
     if constexpr (I < Graph<BG>::rows) {
         constexpr int E = group_end<BG>(I);
         constexpr int P = DUAL ? ahead_in<BG>(I, E) : -1;
@@ -1170,8 +1182,8 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
 
     Messages<msg_slots<BG, DUAL>(), HALF> msg;
     msg.zero();
-    LaneRegs lr{0x80000000u, 0.0f, c.a, c.b};
-    asm volatile("" : "+v"(lr.sgn), "+v"(lr.zero), "+v"(lr.a), "+v"(lr.b));
+    LaneRegs lr{0x80000000u, 0.0f, c.a, c.b, rb + 65536u};
+    asm volatile("" : "+v"(lr.sgn), "+v"(lr.zero), "+v"(lr.a), "+v"(lr.b), "+v"(lr.rbh));
     Ahead ahead;
     bool finished = !lane_ok, restarted = false;
     uint32_t its = 0, ok = 0;
