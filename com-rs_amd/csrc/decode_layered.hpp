@@ -389,6 +389,7 @@ struct LaneRegs {
     float zero;    // +0.0
     float a, b;    // the class's g(m) = a max(m - b, 0)
     uint32_t rbh;  // the lane's row offset + 64 KiB: a base for columns past the LDS offset field
+    uint32_t one;  // 1: the edge counter's step
 };
 
 // Gives the read-ahead values of a layer (edges PRE) an undefined value with no instruction:
@@ -674,12 +675,16 @@ __device__ __forceinline__ void layer(uint32_t gofs, uint32_t& rbp, uint32_t& rb
              g2s = __builtin_amdgcn_bitop3_b32(parity, __float_as_uint(g2), sgn, 0x6c);
     // opaque, or the compiler folds the sign product back into every edge
     asm volatile("" : "+v"(g1s), "+v"(g2s));
-    uint32_t signs = 0, i1 = SPLIT ? 31u : 0u;
+    // the edge number in a vector register (from the zero's bits, one more an edge): the
+    // choice of i1 is then a select of two vector registers
+    uint32_t qv = __float_as_uint(lr.zero);
+    uint32_t signs = 0, i1 = SPLIT ? 31u : qv;
 #pragma unroll
     for (int q = 0; q < deg; ++q) {
         const bool first = fabsf(v2c[q]) == m1;
         const uint32_t sel = first ? g2s : g1s;
-        i1 = first ? uint32_t(q) : i1;
+        if (q > 0) qv += lr.one;
+        i1 = first ? qv : i1;
         // (a & c) ^ b bitwise
         const uint32_t ob = __builtin_amdgcn_bitop3_b32(__float_as_uint(v2c[q]), sel, sgn, 0x6c);
         signs = __builtin_amdgcn_alignbit(signs, ob, 31u);
@@ -1182,8 +1187,8 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
 
     Messages<msg_slots<BG, DUAL>(), HALF> msg;
     msg.zero();
-    LaneRegs lr{0x80000000u, 0.0f, c.a, c.b, rb + 65536u};
-    asm volatile("" : "+v"(lr.sgn), "+v"(lr.zero), "+v"(lr.a), "+v"(lr.b), "+v"(lr.rbh));
+    LaneRegs lr{0x80000000u, 0.0f, c.a, c.b, rb + 65536u, 1u};
+    asm volatile("" : "+v"(lr.sgn), "+v"(lr.zero), "+v"(lr.a), "+v"(lr.b), "+v"(lr.rbh), "+v"(lr.one));
     Ahead ahead;
     bool finished = !lane_ok, restarted = false;
     uint32_t its = 0, ok = 0;
