@@ -31,10 +31,11 @@ LDS_PUBLISHED_GBS = 256 * 128 * 2.4
 VALU_PUBLISHED_GINST = 1024 * 2.4 / 4
 # What the profiles of the plain launch on this workload measured, per codeword
 # (profiles/decode_layered_bg1_z384_16qam.md): HBM bytes from rocprofv3 FETCH_SIZE (x 2.0)
-# and WRITE_SIZE (x 1.0), the channel LLRs in and the decisions out (451.9 MB per 4096
-# codewords); SQ_INSTS_VALU per codeword and iteration (both wave sets, 12 waves)
-TRAFFIC_PER_CODEWORD = 110316
-VALU_PER_CODEWORD_ITERATION = 32231
+# and WRITE_SIZE (x 1.0), the channel LLRs in and the decisions out (453.4 MB per 4096
+# codewords, 1.6 MB of it the stores of two registers spilled in the task set-up);
+# SQ_INSTS_VALU per codeword and iteration (both wave sets, 12 waves)
+TRAFFIC_PER_CODEWORD = 110690
+VALU_PER_CODEWORD_ITERATION = 32144
 PROFILE = "profiles/decode_layered_bg1_z384_16qam.md"
 LEARNED_ITERATIONS = 10
 K_INFO, E, Z, BG1_EDGES = 8448, 25344, 384, 316
