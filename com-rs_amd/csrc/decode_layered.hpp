@@ -308,8 +308,8 @@ __device__ __forceinline__ uint32_t row_pair(int r, int z) {
 // The messages of one lane's rows: output magnitudes g(m1), g(m2) and, packed, the
 // index of the edge that takes g(m2) (bits 0..4) and the output sign bits (edge q of a
 // row at bit 31 - q, so the first edge's is the sign bit itself and each doubling of the word
-// brings the next one there). The index in the low bits is built from inline constants and
-// compared with them, so no register holds an edge number.
+// brings the next one there). The index is compared with inline constants where messages
+// are rebuilt, so no register holds an edge number across the sweep.
 template <int ROWS, bool HALF>
 struct Messages;
 
@@ -379,11 +379,11 @@ struct Ahead {
     uint32_t par;
 };
 
-// Values the sweep keeps in vector registers. A VALU instruction with an operand in a scalar
-// register or an inline constant issues at about half the rate of one whose operands are all
-// vector registers (measured at three waves a SIMD: about 5.3 against 2.7 cycles a wave
-// instruction), so the sign mask, the zero of the offset's clamp and the class's (a, b) are
-// made once per task and held.
+// Values the sweep keeps in vector registers, made once per task. At three waves a SIMD a
+// VALU instruction with an operand in a scalar register or an inline constant issues more
+// slowly than one whose operands are all vector registers, so the constants the layers use
+// on every row or edge are held rather than rebuilt or taken as scalars (each change timed
+// faster on one device against the kernel without it).
 struct LaneRegs {
     uint32_t sgn;  // 0x80000000
     float zero;    // +0.0
