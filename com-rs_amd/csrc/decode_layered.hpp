@@ -247,6 +247,18 @@ __device__ __forceinline__ const gfloat* to_global(const float* p) {
     return reinterpret_cast<const gfloat*>(reinterpret_cast<uintptr_t>(p));
 }
 
+// Values every lane holds alike, made scalar: loads from global memory inside the task
+// loop come back in vector registers (the loop also stores to global memory), and a
+// class's fields in vector registers would cost a register per column base in the sweep.
+__device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uni(float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); }
+template <typename T>
+__device__ __forceinline__ const T* uni(const T* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v)), hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+    return reinterpret_cast<const T*>((uint64_t(hi) << 32) | lo);
+}
+
 // a pointer into LDS made from a byte address
 template <typename T>
 struct lds_ptr {
@@ -759,17 +771,19 @@ __device__ __forceinline__ void group_rest(uint32_t gofs, uint32_t& rbp, uint32_
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL, int I>
 __device__ __forceinline__ void sweep(uint32_t gofs, uint32_t& rbp, uint32_t& rb, bool active, int set,
                                       const uint2* edge, Messages<msg_slots<BG, DUAL>(), HALF>& msg, const gfloat* ab,
-                                      float a, float b, uint32_t xbase, LaneRegs& lr, Ahead& ahead) {
+                                      int st, float a, float b, uint32_t xbase, LaneRegs& lr, Ahead& ahead) {
     if constexpr (I < Graph<BG>::rows) {
         constexpr int E = group_end<BG>(I);
         constexpr int P = DUAL ? ahead_in<BG>(I, E) : -1;
-        // the group's learned parameters, read before the barrier so that their latency
-        // passes while the waves wait there
+        // the group's (a, b), read before the barrier so that their latency passes while the
+        // waves wait there; every iteration reads them, as the learned ones or as the class's
+        // own at stride 0, so that the loads are unconditional and need no default values
         float la[E - I], lb[E - I];
 #pragma unroll
         for (int l = I; l < E; ++l) {
-            la[l - I] = !NMS && ab ? ab[2 * l] : a;
-            lb[l - I] = !NMS && ab ? ab[2 * l + 1] : b;
+            const gfloat* p = NMS ? nullptr : uni(ab + st * l);
+            la[l - I] = NMS ? a : p[0];
+            lb[l - I] = NMS ? b : p[1];
         }
         if constexpr (Sched<BG>::s.barrier[I]) __syncthreads();
         if constexpr (P >= 0)
@@ -786,7 +800,7 @@ __device__ __forceinline__ void sweep(uint32_t gofs, uint32_t& rbp, uint32_t& rb
             else
                 ahead_undefined<Dual<BG>::p.premask[J]>(ahead);
         }
-        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, E>(gofs, rbp, rb, active, set, edge, msg, ab, a, b, xbase, lr, ahead);
+        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, E>(gofs, rbp, rb, active, set, edge, msg, ab, st, a, b, xbase, lr, ahead);
     }
 }
 
@@ -1110,17 +1124,6 @@ __device__ __forceinline__ void place_direct(typename Store<HALF>::T* post, cons
     }
 }
 
-// Values every lane holds alike, made scalar: loads from global memory inside the task
-// loop come back in vector registers (the loop also stores to global memory), and a
-// class's fields in vector registers would cost a register per column base in the sweep.
-__device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
-__device__ __forceinline__ float uni(float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); }
-template <typename T>
-__device__ __forceinline__ const T* uni(const T* p) {
-    const uint64_t v = reinterpret_cast<uint64_t>(p);
-    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v)), hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
-    return reinterpret_cast<const T*>((uint64_t(hi) << 32) | lo);
-}
 __device__ __forceinline__ BatchClass uni(const BatchClass& c) {
     BatchClass u;
     u.bg = uni(c.bg);
@@ -1149,8 +1152,8 @@ __device__ __forceinline__ BatchClass uni(const BatchClass& c) {
 // codeword (FP32), each running its layers on rows 0 .. Z-1 (the second set places no LLRs
 // and writes no outputs).
 template <int BG, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL>
-__device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass& c, uint32_t first, int cnt, Shared& sh,
-                         void* lds_dyn, uint32_t& stamp, uint32_t& round) {
+__device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass& c, const float* own_ab, uint32_t first,
+                                         int cnt, Shared& sh, void* lds_dyn, uint32_t& stamp, uint32_t& round) {
     using S = Store<HALF>;
     using T = typename S::T;
     const int z = ZC ? ZC : c.z, n = ZC ? Graph<BG>::cols * ZC : c.n;
@@ -1242,8 +1245,11 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
         // uniform, and a global-memory pointer: the sweep's reads of the parameters are then
         // vector-memory loads that only the parameters' own use waits for (a generic pointer
         // would make them flat loads, which every barrier's LDS wait waits for too)
-        const gfloat* ab = learned ? to_global(uni(a.params + size_t(it - 1) * 2 * Graph<BG>::rows)) : nullptr;
-        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, 0>(gbytes, rbp, rb, !finished, set, sh.edge, msg, ab, lr.a, lr.b,
+        // (the class's own (a, b) for every layer: stride 0)
+        const gfloat* ab = NMS ? nullptr
+                               : to_global(uni(learned ? a.params + size_t(it - 1) * 2 * Graph<BG>::rows : own_ab));
+        const int st = uni(int32_t(learned ? 2 : 0));
+        sweep<BG, HALF, SOLO, ZC, NMS, DUAL, 0>(gbytes, rbp, rb, !finished, set, sh.edge, msg, ab, st, lr.a, lr.b,
                                                 uint32_t(a.shared_off) + uint32_t(kSharedBytes), lr, ahead);
         __syncthreads();
         const bool fails = fails_now();
@@ -1309,6 +1315,14 @@ __device__ __forceinline__ void layered_body(LayeredArgs a) {
             first = t * uint32_t(c.g);
             cnt = int(min(int64_t(c.g), a.count - int64_t(first)));
         }
+        // the class's (a, b) in global memory, which the sweep reads layer by layer when it has
+        // no learned ones: for a launch of one class, in the kernel's argument segment (the
+        // arguments are the kernel's one parameter, from the segment's first byte; taking the
+        // parameter's own address would copy it to private memory)
+        const float* own_ab =
+            a.tasks ? &a.classes[cls].a
+                    : reinterpret_cast<const float*>(reinterpret_cast<uintptr_t>(__builtin_amdgcn_kernarg_segment_ptr()) +
+                                                     offsetof(LayeredArgs, one) + offsetof(BatchClass, a));
         if (cls != loaded) {
             // the shifts (read with uniform addresses), columns and row starts in LDS
             const int edges = c.bg == 1 ? Graph<1>::edges : Graph<2>::edges;
@@ -1337,10 +1351,10 @@ __device__ __forceinline__ void layered_body(LayeredArgs a) {
             loaded = cls;
         }
         if constexpr ((BGMASK & 1) != 0) {
-            if (c.bg == 1) run_task<1, HALF, SOLO, ZC, NMS, DUAL>(a, c, first, cnt, sh, lds_dyn, stamp, round);
+            if (c.bg == 1) run_task<1, HALF, SOLO, ZC, NMS, DUAL>(a, c, own_ab, first, cnt, sh, lds_dyn, stamp, round);
         }
         if constexpr ((BGMASK & 2) != 0) {
-            if (c.bg == 2) run_task<2, HALF, SOLO, ZC, NMS, DUAL>(a, c, first, cnt, sh, lds_dyn, stamp, round);
+            if (c.bg == 2) run_task<2, HALF, SOLO, ZC, NMS, DUAL>(a, c, own_ab, first, cnt, sh, lds_dyn, stamp, round);
         }
         if (!a.queue) t += gridDim.x;
     }
