@@ -1278,6 +1278,8 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
 // Workgroups take tasks until none is left. BGMASK: the base graphs of the classes; SOLO:
 // every class has G = 1; ZC: every class has Z = ZC (0: any); NMS: every class runs the
 // normalized min-sum and no learned parameters are given; DUAL: two wave sets a codeword.
+// `a` is the calling kernel's only parameter: a launch of one class reads its (a, b) from
+// the kernel's argument segment.
 template <int BGMASK, bool HALF, bool SOLO, int ZC, bool NMS, bool DUAL>
 __device__ __forceinline__ void layered_body(LayeredArgs a) {
     // one dynamic allocation and no static LDS, so that the posteriors start at address 0
@@ -1319,6 +1321,8 @@ __device__ __forceinline__ void layered_body(LayeredArgs a) {
         // no learned ones: for a launch of one class, in the kernel's argument segment (the
         // arguments are the kernel's one parameter, from the segment's first byte; taking the
         // parameter's own address would copy it to private memory)
+        static_assert(offsetof(BatchClass, b) == offsetof(BatchClass, a) + sizeof(float),
+                      "the sweep reads a class's (a, b) as two adjacent floats");
         const float* own_ab =
             a.tasks ? &a.classes[cls].a
                     : reinterpret_cast<const float*>(reinterpret_cast<uintptr_t>(__builtin_amdgcn_kernarg_segment_ptr()) +
