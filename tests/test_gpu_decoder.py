@@ -210,6 +210,26 @@ def test_f32_bg1_z384_bit_exact_with_fp32_oracle(esn0_db):
     assert np.array_equal(bits.cpu().numpy(), (post_ref[:, :8448] < 0).astype(np.uint8))
 
 
+@pytest.mark.parametrize("esn0_db", [4.0, 4.5])
+def test_f32_bg1_z384_offset_min_sum_bit_exact_with_fp32_oracle(esn0_db):
+    """BG1 Z=384, 16-QAM, offset min-sum (beta 0.5): the two-wave-set kernel's general
+    instantiation, which reads the code's (a, b) = (1, beta) in every layer, from the launch's
+    arguments. Posteriors, decisions and iteration counts equal the FP32 oracle's."""
+    print(f"\n[f32 BG1 Z=384 16-QAM OMS 0.5] Es/N0={esn0_db}", flush=True)
+    count = 16
+    ref, infos, llrs = config2_llrs(esn0_db, count)
+    llr32 = llrs.astype(np.float32)
+    full32 = np.array([ref.full_llr(l.astype(np.float64)) for l in llr32]).astype(np.float32)
+    post_ref, it_ref = ref.decode_many_f32(full32, rule=O.OMS, alpha=0.5, layered=True, max_iterations=10,
+                                           threads=8)
+    assert it_ref.min() < 10  # the syndrome stop ends some frames early (at 4.0 dB some reach the limit)
+    code = F.NrLdpc(8448, 25344, 1, 0, 4).set_decoder("offset_min_sum", 0.5, True, 10)
+    bits, its, post = code.decode_batch(cuda(llr32), want_post=True)
+    assert np.array_equal(its.cpu().numpy(), it_ref)
+    assert np.array_equal(post.cpu().numpy(), post_ref)
+    assert np.array_equal(bits.cpu().numpy(), (post_ref[:, :8448] < 0).astype(np.uint8))
+
+
 @pytest.mark.parametrize("esn0_db", [4.0, 4.5, 5.0, 6.0, 7.0])
 def test_f32_against_the_f64_reference_with_recorded_ties(esn0_db):
     """FP32 messages against the reference's f64 decoder on the same LLRs (BG1 Z=384,
