@@ -1191,7 +1191,10 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
     Messages<msg_slots<BG, DUAL>(), HALF> msg;
     msg.zero();
     LaneRegs lr{0x80000000u, 0.0f, c.a, c.b, rb + 65536u, 1u};
-    asm volatile("" : "+v"(lr.sgn), "+v"(lr.zero), "+v"(lr.a), "+v"(lr.b), "+v"(lr.rbh), "+v"(lr.one));
+    // (FP16 kernels leave them to the compiler as constants and scalars: at 168 registers and
+    // four more VALU an edge, the registers they and the edge numbers would hold cost spills)
+    if constexpr (!HALF)
+        asm volatile("" : "+v"(lr.sgn), "+v"(lr.zero), "+v"(lr.a), "+v"(lr.b), "+v"(lr.rbh), "+v"(lr.one));
     Ahead ahead;
     bool finished = !lane_ok, restarted = false;
     uint32_t its = 0, ok = 0;
