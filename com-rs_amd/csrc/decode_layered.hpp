@@ -1191,9 +1191,11 @@ __device__ __forceinline__ void run_task(const LayeredArgs& a, const BatchClass&
     Messages<msg_slots<BG, DUAL>(), HALF> msg;
     msg.zero();
     LaneRegs lr{0x80000000u, 0.0f, c.a, c.b, rb + 65536u, 1u};
-    // (FP16 kernels leave them to the compiler as constants and scalars: at 168 registers and
-    // four more VALU an edge, the registers they and the edge numbers would hold cost spills)
-    if constexpr (!HALF)
+    // (only the two-set kernel, where each was timed faster: the FP16 kernels leave them to
+    // the compiler as constants and scalars, since at 168 registers the registers they and
+    // the edge numbers would hold cost spills, and the one-set FP32 kernels run as fast or
+    // slightly faster without them)
+    if constexpr (DUAL)
         asm volatile("" : "+v"(lr.sgn), "+v"(lr.zero), "+v"(lr.a), "+v"(lr.b), "+v"(lr.rbh), "+v"(lr.one));
     Ahead ahead;
     bool finished = !lane_ok, restarted = false;
