@@ -391,11 +391,12 @@ struct Ahead {
     uint32_t par;
 };
 
-// Values the sweep keeps in vector registers, made once per task. At three waves a SIMD a
-// VALU instruction with an operand in a scalar register or an inline constant issues more
-// slowly than one whose operands are all vector registers, so the constants the layers use
-// on every row or edge are held rather than rebuilt or taken as scalars (each change timed
-// faster on one device against the kernel without it).
+// Values the sweep of the two-set kernel keeps in vector registers, made once per task. At
+// three waves a SIMD a VALU instruction with an operand in a scalar register or an inline
+// constant often issues more slowly than one whose operands are all vector registers, so
+// the constants the layers use on every row or edge are held rather than rebuilt or taken
+// as scalars (each change timed faster on one device against the kernel without it). The
+// other kernels take them as plain values, which the compiler folds into constants.
 struct LaneRegs {
     uint32_t sgn;  // 0x80000000
     float zero;    // +0.0
